@@ -1,0 +1,145 @@
+/*
+ * tdp1375 MI355X proof-search path: C ABI of libtdp1375_jni.so.
+ *
+ * Plain pointers and sizes only.  Every function returns a status code
+ * (TDP_OK on success); on failure tdp_last_error() returns the message of the
+ * calling thread's last error, with the reference's wording
+ * (sorting_search.cu:710-715,842-845; cuda_error.cuh:56-62 with "HIP" for
+ * "CUDA"; cuda_memory.cuh:82-85; pi_filter.cu:165-166).
+ *
+ * The JNI entry points of br.unb.cic.tdp.base.GPUSortingSearch exported by
+ * the same library (jni_shim.cpp) are thin marshalling over these calls.
+ */
+#ifndef TDP1375_H
+#define TDP1375_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#if defined(__GNUC__)
+#define TDP_API __attribute__((visibility("default")))
+#else
+#define TDP_API
+#endif
+
+#define TDP_MAX_N 63
+#define TDP_MAX_DEPTH 15
+#define TDP_FILTER_MAX_OUTPUT (1 << 20)
+
+enum tdp_status {
+    TDP_OK = 0,
+    TDP_ERR_INVALID_ARGUMENT = 1, /* std::invalid_argument in the reference      */
+    TDP_ERR_CANCELLED = 2,        /* GpuSortCancelledException                   */
+    TDP_ERR_QUEUE_OVERFLOW = 3,   /* GpuMemoryExhaustedException (queue overflow) */
+    TDP_ERR_OUT_OF_MEMORY = 4,    /* GpuMemoryExhaustedException (allocation)     */
+    TDP_ERR_HIP = 5,              /* CudaException                               */
+    TDP_ERR_INTERNAL = 6
+};
+
+/* Counters of one search (no counterpart in the reference, which keeps none). */
+typedef struct tdp_counters {
+    uint64_t expansions;  /* popped nodes with depth < D and can_still_reach   */
+    uint64_t generated;   /* children evaluated: C(n,3) per expansion          */
+    uint64_t admitted;    /* non-root children pushed                          */
+    uint64_t dominated;   /* children rejected by the dedup table              */
+    uint64_t improved;    /* admitted because they lowered a stored depth      */
+    uint64_t inserted;    /* admitted as a new table entry                     */
+    uint64_t uninserted;  /* admitted without a table entry (probe limit)      */
+    uint64_t roots_live;  /* roots that are goals or pass can_still_reach      */
+} tdp_counters;
+
+/* Result of one configuration of a batch. */
+typedef struct tdp_case_result {
+    int32_t found;                       /* 1 found, 0 not found              */
+    int32_t depth;                       /* moves in the sequence             */
+    int32_t moves[TDP_MAX_DEPTH][3];     /* positions i<j<k, in move order    */
+    tdp_counters counters;
+} tdp_case_result;
+
+/* Launch statistics of the last search or batch on a slot. */
+typedef struct tdp_launch_stats {
+    double kernel_ms;        /* search kernel time (HIP events)              */
+    int32_t waves;           /* waves launched                               */
+    int32_t overflowed;      /* queue overflow flag                          */
+    uint64_t donations;      /* nodes moved to the shared pool               */
+    uint64_t steals;         /* nodes taken from the shared pool             */
+    uint64_t node_bytes;     /* bytes per node                               */
+} tdp_launch_stats;
+
+/* search flags */
+#define TDP_FLAG_DETERMINISTIC 1u   /* one wave per configuration, no sharing: */
+                                    /* counters are reproducible                */
+#define TDP_FLAG_SINGLE_WAVE   2u   /* the whole launch runs on one wave        */
+
+TDP_API const char *tdp_last_error(void);
+
+/* Replaces initGpuSortSlots (slot_manager.cu:62-98) behind JNI initSlots
+ * (jni_sorter.cu:25-34).  Slot i lives on device i % devices_count. */
+TDP_API int tdp_init_slots(int total_slots, int devices_count, int64_t queue_bytes_budget,
+                   int dedup_table_size);
+
+/* Replaces destroyGpuSortSlots (slot_manager.cu:22-60) behind JNI
+ * destroySlots (jni_sorter.cu:36-43). */
+TDP_API int tdp_destroy_slots(void);
+
+/* Replaces performGpuSortingSearch (sorting_search.cu:701-856) behind JNI
+ * searchForSortingSeq (jni_sorter.cu:45-104).  pi/spi: n bytes each.  On
+ * TDP_OK, *out_depth is the sequence length (0 = none found) and out_moves
+ * holds *out_depth position triples (i,j,k).  Blocks until the search ends or
+ * is cancelled. */
+TDP_API int tdp_search(int slot, const uint8_t *pi, const uint8_t *spi, int n, int initial_even_cycles,
+               float min_rate, int max_moves, int full_sorting, int32_t *out_moves,
+               int32_t *out_depth);
+
+/* tdp_search with flags and counters (extension). */
+TDP_API int tdp_search_ex(int slot, const uint8_t *pi, const uint8_t *spi, int n, int initial_even_cycles,
+                  float min_rate, int max_moves, int full_sorting, uint32_t flags,
+                  int32_t *out_moves, int32_t *out_depth, tdp_counters *out_counters);
+
+/* Many configurations in one persistent launch on one slot (extension; no
+ * reference counterpart).  pis/spis: count x 64 bytes (first n[c] used). */
+TDP_API int tdp_search_batch(int slot, int count, const uint8_t *pis, const uint8_t *spis, const int32_t *ns,
+                     const int32_t *initial_even_cycles, const float *min_rates,
+                     const int32_t *max_moves, const int32_t *full_sorting, uint32_t flags,
+                     tdp_case_result *out_results);
+
+/* Statistics of the last launch on a slot. */
+TDP_API int tdp_last_launch_stats(int slot, tdp_launch_stats *out);
+
+/* Replaces cancelGpuSort (slot_manager.cu:100-125) behind JNI cancelSort
+ * (jni_sorter.cu:106-109).  Never fails. */
+TDP_API void tdp_cancel(int slot);
+
+/* Replaces performGpuPiFilter (pi_filter.cu:157-260) behind JNI
+ * filterPiPermutations (jni_sorter.cu:111-150).  Enumerates the (n-1)!
+ * arrangements with pi[0]=0 in Lehmer-rank order and keeps those passing the
+ * three predicates.  Writes the first min(count, out_rows_cap) passing rows
+ * (n bytes each, rank order) to out; *out_total is the uncapped count. */
+TDP_API int tdp_filter(const uint8_t *cycle, int cycle_len, const uint8_t *triple, int triple_len, int n,
+               int skip_2move_check, uint8_t *out, int64_t out_rows_cap, int64_t *out_total,
+               int64_t *out_written);
+
+/* Kernel time of the last tdp_filter call on this thread, milliseconds. */
+TDP_API double tdp_filter_last_kernel_ms(void);
+
+/* Validation entry (extension): odd-cycle count of each of the C(n,3)
+ * children of (pi, spi), computed on the GPU with the search kernel's LDS
+ * cycle analysis and O(1) parity delta. out: C(n,3) ints. */
+TDP_API int tdp_eval_children(const uint8_t *pi, const uint8_t *spi, int n, int32_t *out);
+
+/* Host-only helpers (no GPU): the same parity delta on the CPU, and the
+ * predicate tables the kernel uses. */
+TDP_API int tdp_host_eval_children(const uint8_t *pi, const uint8_t *spi, int n, int32_t *out);
+TDP_API int tdp_host_predicate_tables(int n, int initial_even_cycles, float min_rate, int max_depth,
+                              int full_sorting, uint64_t *accept16, uint64_t *reach16);
+
+/* Number of visible GPUs (0 when none). */
+TDP_API int tdp_device_count(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,707 @@
+/*
+ * CPU oracle for the tdp1375 proof-search path -- TEST INFRASTRUCTURE ONLY.
+ * Nothing in the product (tdp1375_amd/) links, loads or calls this file; only
+ * tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg do, as the
+ * checker or as the CPU baseline.
+ *
+ * It is a literal, deliberately slow restatement of the reference algorithm
+ * (O(n) cycle count per child, float32 predicates evaluated as written), so
+ * that the HIP path's O(1) parity-delta, integer predicate tables and wave
+ * scan are checked against an independent formulation.
+ *
+ * Parity pins (each function is checked by something other than itself):
+ *   or_count_odd_cycles   : closed forms (identity -> n, n-cycle -> n&1),
+ *                           CycleTest/PermutationGroupsTest conventions, and
+ *                           the reference's case files (every witness replays
+ *                           to the gain the file claims).
+ *   or_satisfies_rate /   : exhaustive comparison against the integer closed
+ *   or_can_still_reach      forms at max_ratio 1.375 (accept <=> g>0 and
+ *                           11g >= 16d, ...), computed in a separate Python
+ *                           restatement in the tests.
+ *   or_apply_move         : CommonOperationsTest.java:21-24 block swap,
+ *                           PermutationGroupsTest.java:9-12 product, replays of
+ *                           all 690 case-file witnesses.
+ *   or_search             : verdicts vs or_bfs_verdict (brute force) on every
+ *                           tested small case; verdict FOUND for every
+ *                           case-file configuration at D >= witness length;
+ *                           NOT FOUND for the bad small components at D=11;
+ *                           returned sequences replay (or_replay_positions).
+ *   or_bfs_verdict        : independent successor generation (cyclic pi
+ *                           rebuilt from the invariant spi o pi = const) and
+ *                           the case files.
+ *   or_java_full_sort_search : verdict equality with or_search(full=1) and
+ *                           or_bfs_verdict(full=1).
+ *   or_filter             : CommonOperationsTest.java:36-46 known answers, a
+ *                           separate Python restatement of the Java predicates
+ *                           (CommonOperations.java:167-182,406-445) on full
+ *                           enumerations for n<=8, and closed-form counts for
+ *                           the trivial inputs.
+ */
+#include "tdp_oracle.h"
+
+#include <math.h>
+#include <pthread.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+
+/* ------------------------------------------------------------------------ */
+/* permutation basics                                                        */
+
+/* sorting_search.cu:302-317 -- number of odd-length cycles ("even" cycles in
+ * the reference's vocabulary, Cycle.java:149-151). */
+int or_count_odd_cycles(const uint8_t *spi, int n) {
+    uint64_t visited = 0;
+    int count = 0;
+    for (int s = 0; s < n; ++s) {
+        if (visited & (1ULL << s)) continue;
+        int size = 0;
+        int cur = s;
+        do {
+            visited |= 1ULL << cur;
+            size++;
+            cur = spi[cur];
+        } while (cur != s);
+        if (size & 1) count++;
+    }
+    return count;
+}
+
+/* common.cuh:25-30 */
+int or_triple_count(int n) {
+    if (n < 3) return 0;
+    return n * (n - 1) * (n - 2) / 6;
+}
+
+/* lexicographic enumeration order of sorting_search.cu:752-758 */
+void or_triple_unrank(int n, int t, int32_t *oi, int32_t *oj, int32_t *ok) {
+    int idx = 0;
+    for (int i = 0; i < n - 2; ++i) {
+        int block = (n - 1 - i) * (n - 2 - i) / 2;
+        if (t >= idx + block) { idx += block; continue; }
+        for (int j = i + 1; j < n - 1; ++j) {
+            int row = n - 1 - j;
+            if (t >= idx + row) { idx += row; continue; }
+            *oi = i; *oj = j; *ok = j + 1 + (t - idx);
+            return;
+        }
+    }
+    *oi = *oj = *ok = -1;
+}
+
+/* transposition.cuh:5-11 */
+static int transposed_src_index(int p, int i, int j, int k) {
+    int lenA = j - i, lenB = k - j;
+    int inLow = (p >= i) & (p < i + lenB);
+    int inHigh = (p >= i + lenB) & (p < k);
+    return p + inLow * lenA - inHigh * lenB;
+}
+
+/* sorting_search.cu:541-555 (spi) and transposition.cuh:13-19 (pi) */
+void or_apply_move(uint8_t *pi, uint8_t *spi, int n, int i, int j, int k) {
+    uint8_t a = pi[i], b = pi[j], c = pi[k];
+    uint8_t av = spi[a], bv = spi[b], cv = spi[c];
+    spi[a] = cv;
+    spi[b] = av;
+    spi[c] = bv;
+    uint8_t tmp[64];
+    for (int p = 0; p < n; ++p) tmp[p] = pi[transposed_src_index(p, i, j, k)];
+    memcpy(pi, tmp, (size_t)n);
+}
+
+/* ------------------------------------------------------------------------ */
+/* predicates, float32 exactly as sorting_search.cu:321-362                  */
+
+int or_satisfies_rate(int even_cycles, int depth, int n, int e0, float min_rate, int full) {
+    if (depth <= 0) return 0;
+    if (full && even_cycles != n) return 0;
+    int gained = even_cycles - e0;
+    volatile float achieved = (float)gained / (float)depth;
+    return gained > 0 && achieved >= min_rate;
+}
+
+int or_can_still_reach(int even_cycles, int depth, int n, int e0, float min_rate,
+                       int max_depth, int full) {
+    int gained = even_cycles - e0;
+    int moves_left = max_depth - depth;
+    if (moves_left < 0) return 0;
+    int max_possible = n - even_cycles;
+    if (full) {
+        int lower = (max_possible + 1) / 2;
+        if (depth + lower > max_depth) return 0;
+        int optimistic = gained + 2 * moves_left;
+        volatile float half = ceilf((float)max_possible / 2.0f);
+        volatile float prod = min_rate * ((float)depth + half);
+        if ((float)optimistic < ceilf(prod)) return 0;
+    } else {
+        int optimistic = gained + 2 * moves_left;
+        volatile float prod = min_rate * (float)max_depth;
+        int required = (int)floorf(prod);
+        if (optimistic < required) return 0;
+        int lower = (max_possible + 1) / 2;
+        int best_depth = depth + lower;
+        if (best_depth > 0) {
+            volatile float ratio = (float)(gained + max_possible) / (float)best_depth;
+            if (ratio < min_rate) return 0;
+        }
+    }
+    return 1;
+}
+
+/* ------------------------------------------------------------------------ */
+/* exact dedup set: spi bytes -> best depth (sorting_search.cu:261-298 with an
+ * unbounded table: no probe limit, no lost insert races)                     */
+
+typedef struct {
+    uint8_t *keys;    /* cap x 64 */
+    uint8_t *depth;   /* cap, 0 = empty */
+    uint64_t cap, used;
+    int n;
+} dedup_set;
+
+static uint64_t key_hash(const uint8_t *k, int n) {
+    uint64_t h = 1469598103934665603ULL;
+    for (int i = 0; i < n; ++i) { h ^= k[i]; h *= 1099511628211ULL; }
+    return h ^ (h >> 29);
+}
+
+static int dedup_init(dedup_set *s, int n) {
+    s->cap = 1u << 12; s->used = 0; s->n = n;
+    s->keys = (uint8_t *)calloc(s->cap, 64);
+    s->depth = (uint8_t *)calloc(s->cap, 1);
+    return s->keys && s->depth ? 0 : -1;
+}
+
+static void dedup_free(dedup_set *s) { free(s->keys); free(s->depth); }
+
+static uint64_t dedup_find_slot(const dedup_set *s, const uint8_t *key) {
+    uint64_t m = s->cap - 1, i = key_hash(key, s->n) & m;
+    while (s->depth[i] && memcmp(s->keys + i * 64, key, (size_t)s->n) != 0) i = (i + 1) & m;
+    return i;
+}
+
+static void dedup_grow(dedup_set *s) {
+    dedup_set t = *s;
+    t.cap = s->cap * 2;
+    t.keys = (uint8_t *)calloc(t.cap, 64);
+    t.depth = (uint8_t *)calloc(t.cap, 1);
+    for (uint64_t i = 0; i < s->cap; ++i) {
+        if (!s->depth[i]) continue;
+        uint64_t j = dedup_find_slot(&t, s->keys + i * 64);
+        memcpy(t.keys + j * 64, s->keys + i * 64, 64);
+        t.depth[j] = s->depth[i];
+    }
+    dedup_free(s);
+    *s = t;
+}
+
+/* returns 0 dominated, 1 improved, 2 inserted */
+static int dedup_admit(dedup_set *s, const uint8_t *key, int depth) {
+    if (2 * (s->used + 1) > s->cap) dedup_grow(s);
+    uint64_t i = dedup_find_slot(s, key);
+    if (s->depth[i]) {
+        if (s->depth[i] <= depth) return 0;
+        s->depth[i] = (uint8_t)depth;
+        return 1;
+    }
+    memset(s->keys + i * 64, 0, 64);
+    memcpy(s->keys + i * 64, key, (size_t)s->n);
+    s->depth[i] = (uint8_t)depth;
+    s->used++;
+    return 2;
+}
+
+/* ------------------------------------------------------------------------ */
+/* deterministic search                                                      */
+
+typedef struct {
+    uint8_t depth, even;
+    uint16_t moves[OR_MAX_DEPTH];
+    uint8_t spi[64];
+    uint8_t pi[64];
+} or_node;
+
+typedef struct { or_node *v; size_t len, cap; } node_stack;
+
+static int stack_push(node_stack *s, const or_node *x) {
+    if (s->len == s->cap) {
+        size_t nc = s->cap ? s->cap * 2 : 1024;
+        or_node *nv = (or_node *)realloc(s->v, nc * sizeof(or_node));
+        if (!nv) return -1;
+        s->v = nv; s->cap = nc;
+    }
+    s->v[s->len++] = *x;
+    return 0;
+}
+
+static void make_child(const or_node *parent, const int32_t *tri, int t, int n, or_node *child) {
+    int i = tri[3 * t], j = tri[3 * t + 1], k = tri[3 * t + 2];
+    memcpy(child->spi, parent->spi, 64);
+    memcpy(child->pi, parent->pi, 64);
+    or_apply_move(child->pi, child->spi, n, i, j, k);
+    memcpy(child->moves, parent->moves, sizeof(child->moves));
+    child->moves[parent->depth] = (uint16_t)t;
+    child->depth = (uint8_t)(parent->depth + 1);
+    child->even = (uint8_t)or_count_odd_cycles(child->spi, n);
+}
+
+/* child odd-cycle count without building the child pi (sorting_search.cu:541-557) */
+static int child_even(const or_node *parent, const int32_t *tri, int t, int n) {
+    uint8_t cs[64];
+    memcpy(cs, parent->spi, (size_t)n);
+    uint8_t a = parent->pi[tri[3 * t]], b = parent->pi[tri[3 * t + 1]], c = parent->pi[tri[3 * t + 2]];
+    uint8_t av = parent->spi[a], bv = parent->spi[b], cv = parent->spi[c];
+    cs[a] = cv; cs[b] = av; cs[c] = bv;
+    return or_count_odd_cycles(cs, n);
+}
+
+int or_search(const uint8_t *pi, const uint8_t *spi, int n, int e0, float min_rate,
+              int max_depth, int full, int use_dedup, uint64_t max_expansions,
+              or_result *out) {
+    memset(out, 0, sizeof(*out));
+    if (n > OR_MAX_N || n < 0) { out->status = -1; return -1; }
+    if (max_depth <= 0 || max_depth > OR_MAX_DEPTH) { out->status = -2; return -2; }
+    const int T = or_triple_count(n);
+    if (T == 0) return 0;
+
+    int32_t *tri = (int32_t *)malloc(sizeof(int32_t) * 3 * (size_t)T);
+    int *ce = (int *)malloc(sizeof(int) * (size_t)T);
+    int t = 0;
+    for (int i = 0; i < n - 2; ++i)
+        for (int j = i + 1; j < n - 1; ++j)
+            for (int k = j + 1; k < n; ++k) { tri[3 * t] = i; tri[3 * t + 1] = j; tri[3 * t + 2] = k; ++t; }
+
+    dedup_set table;
+    if (use_dedup) dedup_init(&table, n);
+    node_stack st = {0};
+
+    or_node start;
+    memset(&start, 0, sizeof(start));
+    memcpy(start.spi, spi, (size_t)n);
+    memcpy(start.pi, pi, (size_t)n);
+    start.depth = 0;
+
+    /* roots: initialize_queues_kernel with Q=1 (sorting_search.cu:403-441):
+     * root r holds triple T-1-r, all admitted at depth 1, popped LIFO. */
+    for (int r = 0; r < T; ++r) {
+        int tt = T - 1 - r;
+        or_node child;
+        make_child(&start, tri, tt, n, &child);
+        if (use_dedup) dedup_admit(&table, child.spi, 1);
+        if (or_satisfies_rate(child.even, 1, n, e0, min_rate, full) ||
+            (1 < max_depth && or_can_still_reach(child.even, 1, n, e0, min_rate, max_depth, full)))
+            out->roots_live++;
+        stack_push(&st, &child);
+    }
+
+    while (st.len) {
+        or_node node = st.v[--st.len];
+        out->pops++;
+        const int d = node.depth;
+        if (or_satisfies_rate(node.even, d, n, e0, min_rate, full)) {
+            out->found = 1;
+            out->depth = d;
+            for (int m = 0; m < d; ++m)
+                or_triple_unrank(n, node.moves[m], &out->moves[m][0], &out->moves[m][1], &out->moves[m][2]);
+            break;
+        }
+        if (d >= max_depth || !or_can_still_reach(node.even, d, n, e0, min_rate, max_depth, full)) continue;
+        if (max_expansions && out->expansions >= max_expansions) { out->aborted = 1; break; }
+        out->expansions++;
+        out->generated += (uint64_t)T;
+        const int cd = d + 1;
+        /* goal on child generation; the reported child is the first goal in
+         * descending triple order (sorting_search.cu:533-535,580-592) */
+        int goal_t = -1;
+        for (int tt = T - 1; tt >= 0; --tt) {
+            ce[tt] = child_even(&node, tri, tt, n);
+            if (goal_t < 0 && or_satisfies_rate(ce[tt], cd, n, e0, min_rate, full)) goal_t = tt;
+        }
+        if (goal_t >= 0) {
+            out->found = 1;
+            out->depth = cd;
+            for (int m = 0; m < d; ++m)
+                or_triple_unrank(n, node.moves[m], &out->moves[m][0], &out->moves[m][1], &out->moves[m][2]);
+            out->moves[d][0] = tri[3 * goal_t];
+            out->moves[d][1] = tri[3 * goal_t + 1];
+            out->moves[d][2] = tri[3 * goal_t + 2];
+            break;
+        }
+        if (cd >= max_depth) continue;
+        /* pushes in descending triple order, so LIFO pops ascend */
+        for (int tt = T - 1; tt >= 0; --tt) {
+            if (!or_can_still_reach(ce[tt], cd, n, e0, min_rate, max_depth, full)) continue;
+            or_node child;
+            make_child(&node, tri, tt, n, &child);
+            if (use_dedup) {
+                int r = dedup_admit(&table, child.spi, cd);
+                if (r == 0) { out->dominated++; continue; }
+                if (r == 1) out->improved++; else out->inserted++;
+            }
+            out->admitted++;
+            stack_push(&st, &child);
+        }
+    }
+
+    free(st.v);
+    if (use_dedup) dedup_free(&table);
+    free(tri);
+    free(ce);
+    return 0;
+}
+
+/* odd-cycle count of every child in lexicographic triple order (O(n) each) */
+int or_children_even(const uint8_t *pi, const uint8_t *spi, int n, int32_t *out) {
+    if (n < 0 || n > OR_MAX_N) return -1;
+    or_node node;
+    memset(&node, 0, sizeof(node));
+    memcpy(node.spi, spi, (size_t)n);
+    memcpy(node.pi, pi, (size_t)n);
+    int t = 0;
+    int32_t tri[3];
+    for (int i = 0; i < n - 2; ++i)
+        for (int j = i + 1; j < n - 1; ++j)
+            for (int k = j + 1; k < n; ++k) {
+                tri[0] = i; tri[1] = j; tri[2] = k;
+                out[t++] = child_even(&node, tri, 0, n);
+            }
+    return t;
+}
+
+/* ------------------------------------------------------------------------ */
+/* multi-threaded batch (CPU baseline)                                       */
+
+typedef struct {
+    int count;
+    const uint8_t *pis, *spis;
+    const int32_t *ns, *e0s, *max_depths, *fulls;
+    const float *min_rates;
+    int use_dedup;
+    uint64_t max_expansions;
+    or_result *outs;
+    int next;
+    pthread_mutex_t mu;
+} batch_job;
+
+static void *batch_worker(void *arg) {
+    batch_job *job = (batch_job *)arg;
+    for (;;) {
+        pthread_mutex_lock(&job->mu);
+        int c = job->next++;
+        pthread_mutex_unlock(&job->mu);
+        if (c >= job->count) break;
+        or_search(job->pis + 64 * (size_t)c, job->spis + 64 * (size_t)c, job->ns[c], job->e0s[c],
+                  job->min_rates[c], job->max_depths[c], job->fulls[c], job->use_dedup,
+                  job->max_expansions, &job->outs[c]);
+    }
+    return NULL;
+}
+
+double or_search_many(int count, const uint8_t *pis, const uint8_t *spis, const int32_t *ns,
+                      const int32_t *e0s, const float *min_rates, const int32_t *max_depths,
+                      const int32_t *fulls, int use_dedup, uint64_t max_expansions,
+                      int threads, or_result *outs) {
+    batch_job job;
+    memset(&job, 0, sizeof(job));
+    job.count = count; job.pis = pis; job.spis = spis; job.ns = ns; job.e0s = e0s;
+    job.max_depths = max_depths; job.fulls = fulls; job.min_rates = min_rates;
+    job.use_dedup = use_dedup; job.max_expansions = max_expansions; job.outs = outs;
+    pthread_mutex_init(&job.mu, NULL);
+    if (threads < 1) threads = 1;
+    pthread_t *th = (pthread_t *)malloc(sizeof(pthread_t) * (size_t)threads);
+    struct timespec t0, t1;
+    clock_gettime(CLOCK_MONOTONIC, &t0);
+    for (int i = 0; i < threads; ++i) pthread_create(&th[i], NULL, batch_worker, &job);
+    for (int i = 0; i < threads; ++i) pthread_join(th[i], NULL);
+    clock_gettime(CLOCK_MONOTONIC, &t1);
+    free(th);
+    pthread_mutex_destroy(&job.mu);
+    return (double)(t1.tv_sec - t0.tv_sec) + 1e-9 * (double)(t1.tv_nsec - t0.tv_nsec);
+}
+
+/* ------------------------------------------------------------------------ */
+/* SortingSearch.java:35-104 (fullSorting CPU search, recursion, no dedup)    */
+
+static int g_java_float_prune = 0;
+
+static int java_rec(uint8_t *pi, uint8_t *spi, int n, int32_t *moves, int depth, int e0,
+                    float min_rate, int max_moves) {
+    const int e = or_count_odd_cycles(spi, n);
+    if (e == n) {
+        int g = e - e0;
+        volatile float ratio = (float)g / (float)depth;
+        if (ratio >= min_rate) return depth;
+        return 0;
+    }
+    if (depth == max_moves) return 0;
+    int g = e - e0;
+    int moves_left = max_moves - depth;
+    int maxp = n - e;
+    /* float minRate * double -> double multiplication (Java promotion).  The
+     * device kernel rounds the same product to float32 (sorting_search.cu:347),
+     * which differs when the product lands just above an integer; the
+     * float-prune variant exists to show that this is the only difference. */
+    double bound;
+    if (g_java_float_prune) {
+        volatile float prod = min_rate * ((float)depth + ceilf((float)maxp / 2.0f));
+        bound = (double)ceilf(prod);
+    } else {
+        bound = ceil((double)min_rate * ((double)depth + ceil((double)maxp / 2.0)));
+    }
+    if ((double)(g + moves_left * 2) < bound) return 0;
+    int lower = (int)ceil((double)maxp / 2.0);
+    if (depth + lower > max_moves) return 0;
+    uint8_t save_pi[64], save_spi[64];
+    for (int i = 0; i < n - 2; ++i)
+        for (int j = i + 1; j < n - 1; ++j)
+            for (int k = j + 1; k < n; ++k) {
+                memcpy(save_pi, pi, (size_t)n);
+                memcpy(save_spi, spi, (size_t)n);
+                moves[depth * 3] = pi[i];
+                moves[depth * 3 + 1] = pi[j];
+                moves[depth * 3 + 2] = pi[k];
+                or_apply_move(pi, spi, n, i, j, k);
+                int r = java_rec(pi, spi, n, moves, depth + 1, e0, min_rate, max_moves);
+                if (r > 0) return r;
+                memcpy(pi, save_pi, (size_t)n);
+                memcpy(spi, save_spi, (size_t)n);
+            }
+    return 0;
+}
+
+int or_java_full_sort_search(const uint8_t *pi, const uint8_t *spi, int n, int e0,
+                             float min_rate, int max_moves, int float_prune,
+                             int32_t *out_symbol_moves) {
+    if (n > OR_MAX_N || max_moves <= 0 || max_moves > OR_MAX_DEPTH) return -1;
+    g_java_float_prune = float_prune;
+    uint8_t p[64], s[64];
+    memcpy(p, pi, (size_t)n);
+    memcpy(s, spi, (size_t)n);
+    return java_rec(p, s, n, out_symbol_moves, 0, e0, min_rate, max_moves);
+}
+
+/* ------------------------------------------------------------------------ */
+/* brute force: BFS over spi with successors from the cyclic pi recovered via  */
+/* the invariant K = spi o pi (pi as successor map)                            */
+
+static uint32_t perm_rank(const uint8_t *p, int n) {
+    uint32_t r = 0;
+    uint16_t used = 0;
+    for (int i = 0; i < n; ++i) {
+        int smaller = 0;
+        for (int v = 0; v < p[i]; ++v) if (!(used & (1u << v))) smaller++;
+        used |= (uint16_t)(1u << p[i]);
+        r = r * (uint32_t)(n - i) + (uint32_t)smaller;
+    }
+    return r;
+}
+
+int or_bfs_verdict(const uint8_t *pi, const uint8_t *spi, int n, int e0, float min_rate,
+                   int max_depth, int full) {
+    if (n > 9 || n < 0) return -1;
+    if (n < 3) return 0;
+    uint32_t total = 1;
+    for (int i = 2; i <= n; ++i) total *= (uint32_t)i;
+    /* K(x) = spi(succ(x)), succ = cyclic successor of the linear pi */
+    uint8_t succ[16], K[16];
+    for (int p = 0; p < n; ++p) succ[pi[p]] = pi[(p + 1) % n];
+    for (int x = 0; x < n; ++x) K[x] = spi[succ[x]];
+    int8_t *dist = (int8_t *)malloc(total);
+    memset(dist, -1, total);
+    uint8_t *queue = (uint8_t *)malloc((size_t)total * (size_t)n);
+    size_t qh = 0, qt = 0;
+    memcpy(queue, spi, (size_t)n);
+    qt = 1;
+    dist[perm_rank(spi, n)] = 0;
+    int verdict = 0;
+    while (qh < qt && !verdict) {
+        const uint8_t *s = queue + qh * (size_t)n;
+        int d = dist[perm_rank(s, n)];
+        qh++;
+        if (d >= max_depth) continue;
+        uint8_t inv[16], sc[16], lin[16];
+        for (int x = 0; x < n; ++x) inv[s[x]] = (uint8_t)x;
+        for (int x = 0; x < n; ++x) sc[x] = inv[K[x]]; /* succ = spi^-1 o K */
+        lin[0] = 0;
+        for (int p = 1; p < n; ++p) lin[p] = sc[lin[p - 1]];
+        for (int i = 0; i < n - 2 && !verdict; ++i)
+            for (int j = i + 1; j < n - 1 && !verdict; ++j)
+                for (int k = j + 1; k < n; ++k) {
+                    uint8_t c[16];
+                    memcpy(c, s, (size_t)n);
+                    uint8_t a = lin[i], b = lin[j], cc = lin[k];
+                    c[a] = s[cc]; c[b] = s[a]; c[cc] = s[b];
+                    uint32_t r = perm_rank(c, n);
+                    if (dist[r] >= 0) continue;
+                    dist[r] = (int8_t)(d + 1);
+                    if (or_satisfies_rate(or_count_odd_cycles(c, n), d + 1, n, e0, min_rate, full)) {
+                        verdict = 1;
+                        break;
+                    }
+                    memcpy(queue + qt * (size_t)n, c, (size_t)n);
+                    qt++;
+                }
+    }
+    free(dist);
+    free(queue);
+    return verdict;
+}
+
+/* ------------------------------------------------------------------------ */
+/* replay                                                                    */
+
+int or_replay_positions(uint8_t *pi, uint8_t *spi, int n, const int32_t *moves, int depth) {
+    for (int m = 0; m < depth; ++m) {
+        int i = moves[3 * m], j = moves[3 * m + 1], k = moves[3 * m + 2];
+        if (!(0 <= i && i < j && j < k && k < n)) return -1;
+        or_apply_move(pi, spi, n, i, j, k);
+    }
+    return or_count_odd_cycles(spi, n);
+}
+
+int or_replay_symbols(uint8_t *pi, uint8_t *spi, int n, const int32_t *moves, int depth) {
+    for (int m = 0; m < depth; ++m) {
+        int pos[3] = {-1, -1, -1};
+        for (int q = 0; q < 3; ++q) {
+            int sym = moves[3 * m + q];
+            for (int p = 0; p < n; ++p) if (pi[p] == sym) pos[q] = p;
+            if (pos[q] < 0) return -1;
+        }
+        /* (x y z) == (y z x) == (z x y): pick the rotation with increasing positions */
+        int ok = 0;
+        for (int r = 0; r < 3 && !ok; ++r) {
+            int i = pos[r], j = pos[(r + 1) % 3], k = pos[(r + 2) % 3];
+            if (i < j && j < k) { or_apply_move(pi, spi, n, i, j, k); ok = 1; }
+        }
+        if (!ok) return -1;
+    }
+    return or_count_odd_cycles(spi, n);
+}
+
+/* ------------------------------------------------------------------------ */
+/* pi filter (pi_filter.cu:38-154)                                           */
+
+/* pi_filter.cu:66-77, indexes by symbol */
+static int in_cyclic_order(const uint8_t *idx, const uint8_t *symbols, int count) {
+    int leaps = 0;
+    for (int i = 0; i < count; ++i) {
+        int cur = idx[symbols[i]];
+        int next = idx[symbols[(i + 1) % count]];
+        if (cur > next) {
+            leaps++;
+            if (leaps > 1) return 0;
+        }
+    }
+    return 1;
+}
+
+/* pi_filter.cu:86-115 */
+static int has_2_move(const uint8_t *pos, const uint8_t *cycle, const uint8_t *cidx, int L) {
+    int before = (L % 2 == 1) ? 1 : 0;
+    for (int i = 0; i < L - 2; ++i)
+        for (int j = i + 1; j < L - 1; ++j)
+            for (int k = j + 1; k < L; ++k) {
+                int a = cycle[i], b = cycle[j], c = cycle[k];
+                int ia = pos[a], ib = pos[b], ic = pos[c];
+                int leaps = (ia > ib) + (ib > ic) + (ic > ia);
+                if (leaps > 1) continue;
+                int d[3][2] = {{a, b}, {b, c}, {c, a}};
+                int after = 0;
+                for (int q = 0; q < 3; ++q) {
+                    int x = cidx[d[q][0]], y = cidx[d[q][1]];
+                    int kk = (y >= x) ? y - x : (L - x) + y;
+                    if (kk % 2 == 1) after++;
+                }
+                if (after - before == 2) return 1;
+            }
+    return 0;
+}
+
+int or_filter_accepts(const uint8_t *pi, int n, const uint8_t *cycle, int cycle_len,
+                      const uint8_t *triple, int triple_len, int skip) {
+    uint8_t pos[64], rpos[64], cidx[64];
+    memset(cidx, 0, sizeof(cidx));
+    for (int i = 0; i < cycle_len; ++i) cidx[cycle[i]] = (uint8_t)i;
+    for (int i = 0; i < n; ++i) pos[pi[i]] = (uint8_t)i;
+    if (!in_cyclic_order(pos, triple, triple_len)) return 0;
+    for (int s = 0; s < n; ++s) rpos[s] = (uint8_t)((n - 1) - pos[s]);
+    if (in_cyclic_order(rpos, cycle, cycle_len)) return 0;
+    if (!skip && has_2_move(pos, cycle, cidx, cycle_len)) return 0;
+    return 1;
+}
+
+/* pi_filter.cu:38-53 */
+static void factoradic_unrank(uint64_t index, uint8_t *perm, int n, const uint64_t *fact) {
+    perm[0] = 0;
+    uint8_t avail[64];
+    for (int i = 0; i < n - 1; ++i) avail[i] = (uint8_t)(i + 1);
+    for (int pos = 0; pos < n - 1; ++pos) {
+        uint64_t f = fact[n - 2 - pos];
+        int choice = (int)(index / f);
+        index %= f;
+        perm[pos + 1] = avail[choice];
+        for (int j = choice; j < n - 2 - pos; ++j) avail[j] = avail[j + 1];
+    }
+}
+
+typedef struct {
+    const uint8_t *cycle, *triple;
+    int L, tl, n, skip;
+    uint64_t lo, hi;
+    const uint64_t *fact;
+    uint8_t *out;       /* local: first cap hits of this range */
+    int64_t cap, count, written;
+} filter_part;
+
+static void *filter_worker(void *arg) {
+    filter_part *p = (filter_part *)arg;
+    uint8_t perm[64];
+    for (uint64_t r = p->lo; r < p->hi; ++r) {
+        factoradic_unrank(r, perm, p->n, p->fact);
+        if (!or_filter_accepts(perm, p->n, p->cycle, p->L, p->triple, p->tl, p->skip)) continue;
+        if (p->written < p->cap) {
+            memcpy(p->out + (size_t)p->written * (size_t)p->n, perm, (size_t)p->n);
+            p->written++;
+        }
+        p->count++;
+    }
+    return NULL;
+}
+
+int64_t or_filter(const uint8_t *cycle, int cycle_len, const uint8_t *triple, int triple_len,
+                  int n, int skip, uint8_t *out, int64_t out_cap, int threads, int64_t *written) {
+    if (n < 1 || n > 21 || cycle_len < 0 || triple_len < 0) return -1;
+    for (int i = 0; i < cycle_len; ++i) if (cycle[i] >= n) return -1;
+    for (int i = 0; i < triple_len; ++i) if (triple[i] >= n) return -1;
+    uint64_t fact[64];
+    fact[0] = 1;
+    for (int i = 1; i < n; ++i) fact[i] = fact[i - 1] * (uint64_t)i;
+    uint64_t total = fact[n - 1];
+    if (threads < 1) threads = 1;
+    if ((uint64_t)threads > total) threads = (int)total;
+    filter_part *parts = (filter_part *)calloc((size_t)threads, sizeof(filter_part));
+    pthread_t *th = (pthread_t *)malloc(sizeof(pthread_t) * (size_t)threads);
+    for (int t = 0; t < threads; ++t) {
+        filter_part *p = &parts[t];
+        p->cycle = cycle; p->triple = triple; p->L = cycle_len; p->tl = triple_len;
+        p->n = n; p->skip = skip; p->fact = fact;
+        p->lo = total * (uint64_t)t / (uint64_t)threads;
+        p->hi = total * (uint64_t)(t + 1) / (uint64_t)threads;
+        p->cap = out_cap;
+        p->out = out_cap > 0 ? (uint8_t *)malloc((size_t)out_cap * (size_t)n) : NULL;
+        pthread_create(&th[t], NULL, filter_worker, p);
+    }
+    int64_t count = 0, w = 0;
+    for (int t = 0; t < threads; ++t) {
+        pthread_join(th[t], NULL);
+        filter_part *p = &parts[t];
+        for (int64_t q = 0; q < p->written && w < out_cap; ++q, ++w)
+            memcpy(out + (size_t)w * (size_t)n, p->out + (size_t)q * (size_t)n, (size_t)n);
+        count += p->count;
+        free(p->out);
+    }
+    free(parts);
+    free(th);
+    if (written) *written = w;
+    return count;
+}

@@ -1,0 +1,934 @@
+// Host engine of the MI355X proof-search path: slots, streams, cancellation,
+// device memory, launch orchestration and the C ABI (include/tdp1375.h).
+//
+// Replaces, for one GPU per slot:
+//   slot_manager.cu:13-175          slots, per-slot streams, cancel protocol
+//   sorting_search.cu:617-699       per-slot buffers, wave count
+//   sorting_search.cu:701-856       per-search orchestration and outcomes
+//   pi_filter.cu:157-260            filter host side
+//   cuda_error.cuh, cuda_memory*.cuh  error taxonomy, OOM retry with backoff
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/tdp1375.h"
+#include "tdp_common.h"
+#include "tdp_filter.h"
+#include "tdp_search.h"
+
+namespace tdp {
+namespace {
+
+// ---------------------------------------------------------------- errors
+// Same taxonomy as cuda_error.cuh:44-76, mapped to status codes at the C ABI.
+class HipException : public std::runtime_error {
+public:
+    HipException(hipError_t code, const char *file, int line)
+        : std::runtime_error(build(code, file, line)) {}
+
+private:
+    static std::string build(hipError_t code, const char *file, int line) {
+        std::ostringstream oss;
+        oss << file << ':' << line << ": HIP error " << hipGetErrorName(code) << " ("
+            << hipGetErrorString(code) << ')';
+        return oss.str();
+    }
+};
+class GpuMemoryExhausted : public std::runtime_error {
+public:
+    explicit GpuMemoryExhausted(const std::string &m) : std::runtime_error(m) {}
+};
+class QueueOverflow : public std::runtime_error {
+public:
+    explicit QueueOverflow(const std::string &m) : std::runtime_error(m) {}
+};
+class Cancelled : public std::runtime_error {
+public:
+    explicit Cancelled(const std::string &m) : std::runtime_error(m) {}
+};
+
+#define HIP_CHECK(x)                                                  \
+    do {                                                              \
+        hipError_t e_ = (x);                                          \
+        if (e_ != hipSuccess) throw HipException(e_, __FILE__, __LINE__); \
+    } while (0)
+
+thread_local std::string g_last_error;
+thread_local double g_filter_ms = 0.0;
+
+int fail(int code, const char *msg) {
+    g_last_error = msg;
+    return code;
+}
+
+template <typename F>
+int guarded(F &&f) {
+    try {
+        f();
+        return TDP_OK;
+    } catch (const Cancelled &e) {
+        return fail(TDP_ERR_CANCELLED, e.what());
+    } catch (const QueueOverflow &e) {
+        return fail(TDP_ERR_QUEUE_OVERFLOW, e.what());
+    } catch (const GpuMemoryExhausted &e) {
+        return fail(TDP_ERR_OUT_OF_MEMORY, e.what());
+    } catch (const HipException &e) {
+        return fail(TDP_ERR_HIP, e.what());
+    } catch (const std::invalid_argument &e) {
+        return fail(TDP_ERR_INVALID_ARGUMENT, e.what());
+    } catch (const std::exception &e) {
+        return fail(TDP_ERR_INTERNAL, e.what());
+    }
+}
+
+// ---------------------------------------------------------------- memory
+// cuda_memory.cuh:60-99: retry hipMalloc on out-of-memory with exponential
+// backoff (100 ms doubling to 5 s) until 60 s have passed.
+std::atomic<int64_t> g_alloc_timeout_ms{60000};
+
+void *device_alloc(size_t bytes, const char *label) {
+    if (bytes == 0) return nullptr;
+    void *p = nullptr;
+    const auto start = std::chrono::steady_clock::now();
+    int backoff = 100;
+    while (true) {
+        const hipError_t st = hipMalloc(&p, bytes);
+        if (st == hipSuccess) return p;
+        if (st != hipErrorOutOfMemory) HIP_CHECK(st);
+        (void)hipGetLastError();
+        const auto el = std::chrono::duration_cast<std::chrono::milliseconds>(
+                            std::chrono::steady_clock::now() - start).count();
+        if (el >= g_alloc_timeout_ms.load()) {
+            throw GpuMemoryExhausted(std::string("HIP out of memory while allocating ") +
+                                     std::to_string(bytes) + " bytes for " + label);
+        }
+        std::this_thread::sleep_for(std::chrono::milliseconds(backoff));
+        backoff = std::min(backoff * 2, 5000);
+    }
+}
+
+struct DevFree {
+    int device;
+    void operator()(void *p) const noexcept {
+        if (!p) return;
+        int cur = -1;
+        (void)hipGetDevice(&cur);
+        if (cur != device) (void)hipSetDevice(device);
+        (void)hipFree(p);
+        if (cur >= 0 && cur != device) (void)hipSetDevice(cur);
+    }
+};
+using DevPtr = std::unique_ptr<void, DevFree>;
+
+DevPtr make_dev(size_t bytes, const char *label) {
+    int dev = 0;
+    HIP_CHECK(hipGetDevice(&dev));
+    return DevPtr(device_alloc(bytes, label), DevFree{dev});
+}
+
+struct HostFree {
+    void operator()(void *p) const noexcept {
+        if (p) (void)hipHostFree(p);
+    }
+};
+using HostPtr = std::unique_ptr<void, HostFree>;
+
+HostPtr make_pinned(size_t bytes) {
+    void *p = nullptr;
+    HIP_CHECK(hipHostMalloc(&p, bytes, hipHostMallocDefault));
+    return HostPtr(p);
+}
+
+template <typename T>
+T *as(const DevPtr &p) { return static_cast<T *>(p.get()); }
+template <typename T>
+T *as(const HostPtr &p) { return static_cast<T *>(p.get()); }
+
+uint32_t floor_pow2(uint64_t v) {
+    if (v == 0) return 0;
+    uint64_t p = 1;
+    while (p * 2 <= v && p < (1ULL << 31)) p *= 2;
+    return (uint32_t)p;
+}
+
+// ---------------------------------------------------------------- predicates
+// Bit tables of the reference's float32 predicates (sorting_search.cu:321-362)
+// evaluated on the host: accept[d] bit e <=> satisfies_rate(e, d),
+// reach[d] bit e <=> can_still_reach(e, d).  The arithmetic is IEEE binary32
+// as on the device (no contraction: each product is rounded before use).
+bool satisfies_rate_f32(int e, int d, int n, int e0, float min_rate, bool full) {
+    if (d <= 0) return false;
+    if (full && e != n) return false;
+    const int gained = e - e0;
+    volatile float ratio = (float)gained / (float)d;
+    return gained > 0 && ratio >= min_rate;
+}
+
+bool can_still_reach_f32(int e, int d, int n, int e0, float min_rate, int D, bool full) {
+    const int gained = e - e0;
+    const int moves_left = D - d;
+    if (moves_left < 0) return false;
+    const int maxp = n - e;
+    if (full) {
+        const int lower = (maxp + 1) / 2;
+        if (d + lower > D) return false;
+        const int optimistic = gained + 2 * moves_left;
+        volatile float half = std::ceil((float)maxp / 2.0f);
+        volatile float prod = min_rate * ((float)d + half);
+        if ((float)optimistic < std::ceil(prod)) return false;
+    } else {
+        const int optimistic = gained + 2 * moves_left;
+        volatile float prod = min_rate * (float)D;
+        const int required = (int)std::floor(prod);
+        if (optimistic < required) return false;
+        const int lower = (maxp + 1) / 2;
+        const int best = d + lower;
+        if (best > 0) {
+            volatile float ratio = (float)(gained + maxp) / (float)best;
+            if (ratio < min_rate) return false;
+        }
+    }
+    return true;
+}
+
+void predicate_tables(int n, int e0, float min_rate, int D, bool full, uint64_t *acc, uint64_t *rch) {
+    for (int d = 0; d < 16; ++d) {
+        uint64_t a = 0, r = 0;
+        for (int e = 0; e <= n && e < 64; ++e) {
+            if (satisfies_rate_f32(e, d, n, e0, min_rate, full)) a |= 1ULL << e;
+            if (can_still_reach_f32(e, d, n, e0, min_rate, D, full)) r |= 1ULL << e;
+        }
+        acc[d] = a;
+        rch[d] = r;
+    }
+}
+
+int count_odd_cycles(const uint8_t *spi, int n) {
+    uint64_t seen = 0;
+    int c = 0;
+    for (int s = 0; s < n; ++s) {
+        if (seen >> s & 1) continue;
+        int len = 0, x = s;
+        do {
+            seen |= 1ULL << x;
+            len++;
+            x = spi[x];
+        } while (x != s);
+        c += len & 1;
+    }
+    return c;
+}
+
+// ---------------------------------------------------------------- triples
+struct TripleTable {
+    std::vector<uint32_t> packed;   // all n in 3..63, concatenated
+    uint32_t offset[64] = {0};
+    TripleTable() {
+        for (int n = 0; n <= kMaxN; ++n) {
+            offset[n] = (uint32_t)packed.size();
+            for (int i = 0; i < n - 2; ++i)
+                for (int j = i + 1; j < n - 1; ++j)
+                    for (int k = j + 1; k < n; ++k) packed.push_back(pack_triple(i, j, k));
+        }
+    }
+    void unpack(int n, uint32_t t, int32_t *ijk) const {
+        const uint32_t p = packed[offset[n] + t];
+        ijk[0] = p & 0xFF;
+        ijk[1] = (p >> 8) & 0xFF;
+        ijk[2] = (p >> 16) & 0xFF;
+    }
+};
+const TripleTable &triples() {
+    static TripleTable t;
+    return t;
+}
+
+struct DeviceShared {
+    DevPtr triples;
+    int waves_max = 0;   // resident waves of search_kernel
+    int cus = 0;
+};
+std::mutex g_dev_mutex;
+std::vector<std::unique_ptr<DeviceShared>> g_dev;
+
+DeviceShared &device_shared(int device) {
+    std::lock_guard<std::mutex> lk(g_dev_mutex);
+    if ((size_t)device >= g_dev.size()) g_dev.resize(device + 1);
+    if (!g_dev[device]) {
+        auto d = std::make_unique<DeviceShared>();
+        const auto &tt = triples();
+        d->triples = make_dev(tt.packed.size() * 4, "sort.triples");
+        HIP_CHECK(hipMemcpy(d->triples.get(), tt.packed.data(), tt.packed.size() * 4, hipMemcpyHostToDevice));
+        HIP_CHECK(hipDeviceGetAttribute(&d->cus, hipDeviceAttributeMultiprocessorCount, device));
+        int blocks = 0;
+        HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, search_kernel, 256, 0));
+        d->waves_max = std::max(1, blocks) * d->cus * 4;
+        g_dev[device] = std::move(d);
+    }
+    return *g_dev[device];
+}
+
+// ---------------------------------------------------------------- slots
+struct Slot {
+    int device = -1;
+    hipStream_t stream = nullptr;
+    hipStream_t cancel_stream = nullptr;
+    DevPtr cancel_flag;        // int on the device, polled by the kernel
+    HostPtr cancel_one;        // pinned int holding 1 (slot_manager.cu:86-87)
+    std::atomic<bool> cancelled{false};
+    std::mutex run_mutex;      // one search at a time per slot
+
+    int64_t queue_budget = 0;
+    int64_t dedup_size = 0;
+    uint32_t epoch = 0;
+
+    DevPtr queue_mem;          // stacks + pool nodes
+    size_t queue_bytes = 0;
+    DevPtr pool_seq;
+    size_t pool_seq_cap = 0;
+    DevPtr ctrl;
+    DevPtr dd_tag, dd_best, dd_cfg, dd_key;
+
+    DevPtr cfgs, results;
+    HostPtr h_cfgs, h_results, h_ctrl;
+    size_t cfg_cap = 0;
+
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    tdp_launch_stats stats{};
+};
+
+std::mutex g_slots_mutex;
+std::vector<std::unique_ptr<Slot>> g_slots;
+
+Slot &get_slot(int slot) {
+    std::lock_guard<std::mutex> lk(g_slots_mutex);
+    if (slot < 0 || (size_t)slot >= g_slots.size() || !g_slots[slot]) throw std::invalid_argument("invalid slot");
+    return *g_slots[slot];
+}
+
+void destroy_slot(Slot &s) {
+    if (s.device >= 0) (void)hipSetDevice(s.device);
+    if (s.stream) (void)hipStreamSynchronize(s.stream);
+    if (s.cancel_stream) (void)hipStreamSynchronize(s.cancel_stream);
+    s.queue_mem.reset();
+    s.pool_seq.reset();
+    s.ctrl.reset();
+    s.dd_tag.reset();
+    s.dd_best.reset();
+    s.dd_cfg.reset();
+    s.dd_key.reset();
+    s.cfgs.reset();
+    s.results.reset();
+    s.cancel_flag.reset();
+    s.h_cfgs.reset();
+    s.h_results.reset();
+    s.h_ctrl.reset();
+    s.cancel_one.reset();
+    if (s.ev0) (void)hipEventDestroy(s.ev0);
+    if (s.ev1) (void)hipEventDestroy(s.ev1);
+    if (s.stream) (void)hipStreamDestroy(s.stream);
+    if (s.cancel_stream) (void)hipStreamDestroy(s.cancel_stream);
+    s.stream = s.cancel_stream = nullptr;
+    s.ev0 = s.ev1 = nullptr;
+}
+
+void destroy_all_slots() {
+    std::lock_guard<std::mutex> lk(g_slots_mutex);
+    int cur = -1;
+    (void)hipGetDevice(&cur);
+    for (auto &s : g_slots)
+        if (s) destroy_slot(*s);
+    g_slots.clear();
+    if (cur >= 0) (void)hipSetDevice(cur);
+}
+
+void init_slots(int total, int devices, int64_t budget, int64_t dedup) {
+    if (total <= 0) throw std::invalid_argument("totalSlots must be positive");
+    if (devices <= 0) throw std::invalid_argument("devicesCount must be positive");
+    if (budget < 0) throw std::invalid_argument("queueBytesBudget must be non-negative");
+    destroy_all_slots();
+    int count = 0;
+    HIP_CHECK(hipGetDeviceCount(&count));
+    std::lock_guard<std::mutex> lk(g_slots_mutex);
+    int cur = -1;
+    (void)hipGetDevice(&cur);
+    for (int i = 0; i < total; ++i) {
+        auto s = std::make_unique<Slot>();
+        s->device = i % devices;
+        HIP_CHECK(hipSetDevice(s->device));
+        HIP_CHECK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+        HIP_CHECK(hipStreamCreateWithFlags(&s->cancel_stream, hipStreamNonBlocking));
+        HIP_CHECK(hipEventCreate(&s->ev0));
+        HIP_CHECK(hipEventCreate(&s->ev1));
+        s->cancel_flag = make_dev(sizeof(int), "sort.cancelFlag");
+        HIP_CHECK(hipMemset(s->cancel_flag.get(), 0, sizeof(int)));
+        s->cancel_one = make_pinned(sizeof(int));
+        *as<int>(s->cancel_one) = 1;
+        s->queue_budget = budget;
+        s->dedup_size = std::max<int64_t>(0, dedup);
+        (void)device_shared(s->device);
+        s->queue_bytes = (size_t)budget;
+        s->queue_mem = make_dev(std::max<size_t>(s->queue_bytes, 256), "sort.nodes");
+        s->ctrl = make_dev(sizeof(ControlLine) * kNumControl, "sort.control");
+        s->h_ctrl = make_pinned(sizeof(ControlLine) * kNumControl);
+        if (s->dedup_size > 0) {
+            const size_t cap = (size_t)s->dedup_size;
+            s->dd_tag = make_dev(cap * 8, "sort.dedupTable");
+            s->dd_best = make_dev(cap * 4, "sort.dedupTable");
+            s->dd_cfg = make_dev(cap * 4, "sort.dedupTable");
+            s->dd_key = make_dev(cap * 64, "sort.dedupTable");
+            HIP_CHECK(hipMemset(s->dd_tag.get(), 0, cap * 8));
+            HIP_CHECK(hipMemset(s->dd_best.get(), 0, cap * 4));
+        }
+        s->epoch = 0;
+        g_slots.push_back(std::move(s));
+    }
+    if (cur >= 0) (void)hipSetDevice(cur);
+}
+
+void ensure_config_capacity(Slot &s, size_t count) {
+    if (count <= s.cfg_cap) return;
+    size_t cap = std::max<size_t>(count, 64);
+    s.cfgs = make_dev(cap * sizeof(ConfigDesc), "sort.configs");
+    s.results = make_dev(cap * sizeof(ConfigResult), "sort.results");
+    s.h_cfgs = make_pinned(cap * sizeof(ConfigDesc));
+    s.h_results = make_pinned(cap * sizeof(ConfigResult));
+    s.cfg_cap = cap;
+}
+
+struct CaseInput {
+    const uint8_t *pi, *spi;
+    int n, e0, D, full;
+    float min_rate;
+};
+
+void validate_case(const CaseInput &c) {
+    if (c.n > kMaxN) throw std::invalid_argument("performGpuSortingSearch: n exceeds MAX_N");
+    if (c.D <= 0 || c.D > kMaxDepth) throw std::invalid_argument("performGpuSortingSearch: invalid maximumDepth");
+    if (c.n < 0) throw std::invalid_argument("performGpuSortingSearch: inconsistent input sizes");
+    // The kernel indexes LDS tables by symbol: both inputs must be permutations
+    // of 0..n-1 (the reference leaves this undefined on the device).
+    uint64_t seen_pi = 0, seen_spi = 0;
+    for (int i = 0; i < c.n; ++i) {
+        if (c.pi[i] >= c.n || c.spi[i] >= c.n)
+            throw std::invalid_argument("performGpuSortingSearch: pi and spi must be permutations of 0..n-1");
+        seen_pi |= 1ULL << c.pi[i];
+        seen_spi |= 1ULL << c.spi[i];
+    }
+    const uint64_t full_mask = (c.n == 64) ? ~0ULL : ((1ULL << c.n) - 1ULL);
+    if (seen_pi != full_mask || seen_spi != full_mask)
+        throw std::invalid_argument("performGpuSortingSearch: pi and spi must be permutations of 0..n-1");
+}
+
+void fill_config(ConfigDesc &d, const CaseInput &c) {
+    std::memset(&d, 0, sizeof(d));
+    d.n = (uint32_t)c.n;
+    d.max_depth = (uint32_t)c.D;
+    d.e_start = (uint32_t)count_odd_cycles(c.spi, c.n);
+    d.triple_count = (uint32_t)triple_count(c.n);
+    d.triple_offset = triples().offset[c.n];
+    d.flags = c.full ? 1u : 0u;
+    predicate_tables(c.n, c.e0, c.min_rate, c.D, c.full != 0, d.accept, d.reach);
+    std::memcpy(d.spi0, c.spi, (size_t)c.n);
+    std::memcpy(d.pi0, c.pi, (size_t)c.n);
+}
+
+bool cancel_requested(Slot &s) { return s.cancelled.load(std::memory_order_acquire); }
+
+// Device watchdog: a launch that runs longer than TDP_KERNEL_TIMEOUT_S seconds
+// (default 3600) stops itself and reports an error instead of running on.
+unsigned long long kernel_timeout_ticks() {
+    double secs = 3600.0;
+    if (const char *e = std::getenv("TDP_KERNEL_TIMEOUT_S")) secs = std::atof(e);
+    if (secs <= 0) return 0ULL;
+    return (unsigned long long)(secs * 1e8);   // s_memrealtime runs at 100 MHz
+}
+
+// One persistent launch over `cases` on slot s.  Returns per-case results in
+// s.h_results; throws on cancel / overflow.
+void run_launch(Slot &s, const std::vector<CaseInput> &cases, uint32_t flags, bool single_search) {
+    const int count = (int)cases.size();
+    HIP_CHECK(hipSetDevice(s.device));
+    DeviceShared &dev = device_shared(s.device);
+    ensure_config_capacity(s, (size_t)count);
+    int max_n = 3;
+    for (int c = 0; c < count; ++c) {
+        fill_config(as<ConfigDesc>(s.h_cfgs)[c], cases[c]);
+        max_n = std::max(max_n, cases[c].n);
+    }
+    const uint32_t nw = (uint32_t)node_words_for(max_n);
+    const size_t node_bytes = (size_t)nw * 4;
+
+    int waves;
+    bool donate;
+    if (flags & TDP_FLAG_SINGLE_WAVE) {
+        waves = 1;
+        donate = false;
+    } else if (flags & TDP_FLAG_DETERMINISTIC) {
+        waves = std::min(count, dev.waves_max);
+        donate = false;
+    } else {
+        waves = dev.waves_max;
+        donate = true;
+    }
+    // queue memory: 3/4 private stacks, 1/4 shared pool (sorting_search.cu:655-658)
+    size_t pool_bytes = donate ? s.queue_bytes / 4 : 0;
+    size_t stack_bytes = s.queue_bytes - pool_bytes;
+    uint32_t stack_cap = floor_pow2(stack_bytes / ((size_t)waves * node_bytes));
+    if (stack_cap < 2) {
+        // the budget cannot hold two nodes per wave: fewer waves
+        waves = std::max<int>(1, (int)(stack_bytes / (2 * node_bytes)));
+        stack_cap = floor_pow2(stack_bytes / ((size_t)std::max(1, waves) * node_bytes));
+    }
+    if (stack_cap < 2) throw QueueOverflow("GPU sorting search queue overflow");
+    uint32_t pool_cap = donate ? std::min<uint32_t>(floor_pow2(pool_bytes / node_bytes), 1u << 20) : 0;
+    if (donate && pool_cap < 64) {
+        donate = false;
+        pool_cap = 0;
+    }
+    const size_t pool_cap_alloc = std::max<size_t>(pool_cap, 1);
+    if (s.pool_seq_cap < pool_cap_alloc) {
+        s.pool_seq = make_dev(pool_cap_alloc * 8, "sort.poolSeq");
+        s.pool_seq_cap = pool_cap_alloc;
+    }
+
+    // dedup epoch (replaces the per-search table memset, sorting_search.cu:744-747)
+    if (s.dedup_size > 0) {
+        s.epoch++;
+        if (s.epoch > 0xFFFF) {
+            HIP_CHECK(hipMemsetAsync(s.dd_tag.get(), 0, (size_t)s.dedup_size * 8, s.stream));
+            HIP_CHECK(hipMemsetAsync(s.dd_best.get(), 0, (size_t)s.dedup_size * 4, s.stream));
+            s.epoch = 1;
+        }
+    }
+
+    SearchParams P{};
+    P.cfgs = as<ConfigDesc>(s.cfgs);
+    P.results = as<ConfigResult>(s.results);
+    P.num_configs = count;
+    P.num_waves = waves;
+    P.triples = as<uint32_t>(dev.triples);
+    P.stacks = as<uint32_t>(s.queue_mem);
+    P.stack_cap = stack_cap;
+    P.node_words = nw;
+    P.spi_words = (uint32_t)spi_words_for(max_n);
+    P.pool_nodes = P.stacks + (size_t)waves * stack_cap * nw;
+    P.pool_seq = as<unsigned long long>(s.pool_seq);
+    P.pool_cap = pool_cap ? pool_cap : 1;
+    P.ctrl = as<ControlLine>(s.ctrl);
+    P.dd_cap = s.dedup_size > 0 ? (unsigned long long)s.dedup_size : 0ULL;
+    P.dd_tag = as<unsigned long long>(s.dd_tag);
+    P.dd_best = as<uint32_t>(s.dd_best);
+    P.dd_cfg = as<uint32_t>(s.dd_cfg);
+    P.dd_key = as<uint32_t>(s.dd_key);
+    P.epoch = s.epoch ? s.epoch : 1;
+    P.cancel_flag = as<int>(s.cancel_flag);
+    P.donate = donate ? 1 : 0;
+    P.timeout_ticks = kernel_timeout_ticks();
+
+    HIP_CHECK(hipMemcpyAsync(s.cfgs.get(), s.h_cfgs.get(), (size_t)count * sizeof(ConfigDesc),
+                             hipMemcpyHostToDevice, s.stream));
+    const size_t reset_n = std::max<size_t>({(size_t)count, (size_t)P.pool_cap, (size_t)kNumControl});
+    hipLaunchKernelGGL(reset_kernel, dim3((unsigned)((reset_n + 255) / 256)), dim3(256), 0, s.stream, P);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipEventRecord(s.ev0, s.stream));
+    const int blocks = (waves + 3) / 4;
+    hipLaunchKernelGGL(search_kernel, dim3(blocks), dim3(256), 0, s.stream, P);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipEventRecord(s.ev1, s.stream));
+    HIP_CHECK(hipMemcpyAsync(s.h_results.get(), s.results.get(), (size_t)count * sizeof(ConfigResult),
+                             hipMemcpyDeviceToHost, s.stream));
+    HIP_CHECK(hipMemcpyAsync(s.h_ctrl.get(), s.ctrl.get(), sizeof(ControlLine) * kNumControl,
+                             hipMemcpyDeviceToHost, s.stream));
+
+    // wait: short spin, then poll with a small sleep; stop polling once a
+    // cancel was requested (the kernel sees the device flag and drains)
+    const auto t0 = std::chrono::steady_clock::now();
+    while (!cancel_requested(s)) {
+        const hipError_t st = hipStreamQuery(s.stream);
+        if (st == hipSuccess) break;
+        if (st != hipErrorNotReady) HIP_CHECK(st);
+        const auto us = std::chrono::duration_cast<std::chrono::microseconds>(
+                            std::chrono::steady_clock::now() - t0).count();
+        if (us > 200) std::this_thread::sleep_for(std::chrono::microseconds(us > 20000 ? 500 : 50));
+    }
+    HIP_CHECK(hipStreamSynchronize(s.stream));
+    float ms = 0.f;
+    HIP_CHECK(hipEventElapsedTime(&ms, s.ev0, s.ev1));
+    const ControlLine *ctl = as<ControlLine>(s.h_ctrl);
+    s.stats.kernel_ms = ms;
+    s.stats.waves = waves;
+    s.stats.overflowed = ctl[kCtlStop].v != 0;
+    s.stats.donations = ctl[kCtlDonations].v;
+    s.stats.steals = ctl[kCtlSteals].v;
+    s.stats.node_bytes = node_bytes;
+
+    const ConfigResult *res = as<ConfigResult>(s.h_results);
+    bool all_found = true;
+    for (int c = 0; c < count; ++c) all_found &= res[c].found != 0;
+    const bool cancelled = cancel_requested(s);
+    if (ctl[kCtlStop].v == 2)
+        throw std::runtime_error("GPU sorting search watchdog timeout (TDP_KERNEL_TIMEOUT_S)");
+    // sorting_search.cu:841-846
+    if (!all_found && cancelled) throw Cancelled("GPU sorting search cancelled");
+    if (!all_found && ctl[kCtlStop].v != 0) throw QueueOverflow("GPU sorting search queue overflow");
+    (void)single_search;
+}
+
+void copy_counters(const ConfigResult &r, tdp_counters *out) {
+    out->expansions = r.counters[kCntExpansions];
+    out->generated = r.counters[kCntGenerated];
+    out->admitted = r.counters[kCntAdmitted];
+    out->dominated = r.counters[kCntDominated];
+    out->improved = r.counters[kCntImproved];
+    out->inserted = r.counters[kCntInserted];
+    out->uninserted = r.counters[kCntUninserted];
+    out->roots_live = r.counters[kCntRootsLive];
+}
+
+void clear_cancel(Slot &s) {
+    // slot_manager.cu:127-144: clear the host flag, let a pending cancel copy
+    // land, then clear the device flag on the slot stream
+    s.cancelled.store(false, std::memory_order_release);
+    HIP_CHECK(hipStreamSynchronize(s.cancel_stream));
+    HIP_CHECK(hipMemsetAsync(s.cancel_flag.get(), 0, sizeof(int), s.stream));
+}
+
+int search_impl(int slot, const uint8_t *pi, const uint8_t *spi, int n, int e0, float min_rate, int D,
+                int full, uint32_t flags, int32_t *out_moves, int32_t *out_depth, tdp_counters *out_counters) {
+    return guarded([&] {
+        if (!pi || !spi || !out_moves || !out_depth) throw std::invalid_argument("null argument");
+        CaseInput c{pi, spi, n, e0, D, full, min_rate};
+        if (n > kMaxN) throw std::invalid_argument("performGpuSortingSearch: n exceeds MAX_N");
+        if (D <= 0 || D > kMaxDepth) throw std::invalid_argument("performGpuSortingSearch: invalid maximumDepth");
+        Slot &s = get_slot(slot);
+        std::lock_guard<std::mutex> run(s.run_mutex);
+        HIP_CHECK(hipSetDevice(s.device));
+        clear_cancel(s);
+        *out_depth = 0;
+        if (out_counters) std::memset(out_counters, 0, sizeof(*out_counters));
+        if (triple_count(n) == 0) return;   // sorting_search.cu:718-719
+        validate_case(c);
+        std::vector<CaseInput> cases{c};
+        run_launch(s, cases, flags, true);
+        const ConfigResult &r = as<ConfigResult>(s.h_results)[0];
+        if (out_counters) copy_counters(r, out_counters);
+        if (!r.found || r.depth <= 0) return;
+        for (int m = 0; m < r.depth; ++m) triples().unpack(n, r.moves[m], out_moves + 3 * m);
+        *out_depth = r.depth;
+    });
+}
+
+// ---------------------------------------------------------------- filter
+struct FilterWorkspace {
+    int device = -1;
+    DevPtr mask, counts, offsets, total, out, moves;
+    size_t mask_words = 0, blocks = 0, out_bytes = 0, moves_cap = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+std::mutex g_filter_mutex;
+std::vector<std::unique_ptr<FilterWorkspace>> g_filter_ws;
+
+void filter_impl(const uint8_t *cycle, int L, const uint8_t *triple, int tl, int n, int skip, uint8_t *out,
+                 int64_t out_cap, int64_t *out_total, int64_t *out_written) {
+    // pi_filter.cu:165-166
+    if (n > kMaxN) throw std::invalid_argument("n exceeds PI_FILTER_MAX_N");
+    if (n < 1) throw std::invalid_argument("n must be at least 1");
+    if (n > 21) throw std::invalid_argument("n too large: (n-1)! exceeds 64-bit rank enumeration");
+    if (L < 0 || tl < 0 || L > 64 || tl > 64) throw std::invalid_argument("invalid cycle or triple length");
+    if ((L > 0 && !cycle) || (tl > 0 && !triple)) throw std::invalid_argument("null argument");
+    for (int i = 0; i < L; ++i)
+        if (cycle[i] >= n) throw std::invalid_argument("cycle symbol out of range");
+    for (int i = 0; i < tl; ++i)
+        if (triple[i] >= n) throw std::invalid_argument("triple symbol out of range");
+    if (out_cap < 0) throw std::invalid_argument("negative output capacity");
+
+    FilterParams F{};
+    F.n = n;
+    F.cycle_len = L;
+    F.triple_len = tl;
+    F.skip_2move = skip ? 1 : 0;
+    std::memcpy(F.cycle, cycle, (size_t)L);
+    std::memcpy(F.triple, triple, (size_t)tl);
+    // 2-move candidates of the cycle (pi_filter.cu:86-115): triples of cycle
+    // indices i<j<k whose arcs (taken through the symbol -> index map as the
+    // reference does) turn `before` into `before + 2` odd cycles.
+    uint8_t cidx[64] = {0};
+    for (int i = 0; i < L; ++i) cidx[cycle[i]] = (uint8_t)i;
+    auto dist = [&](int a, int b) {
+        const int ai = cidx[a], bi = cidx[b];
+        return bi >= ai ? bi - ai : (L - ai) + bi;
+    };
+    std::vector<uint32_t> moves;
+    const int before = (L % 2 == 1) ? 1 : 0;
+    for (int i = 0; i < L - 2; ++i)
+        for (int j = i + 1; j < L - 1; ++j)
+            for (int k = j + 1; k < L; ++k) {
+                const int a = cycle[i], b = cycle[j], c = cycle[k];
+                const int after = (dist(a, b) % 2 == 1) + (dist(b, c) % 2 == 1) + (dist(c, a) % 2 == 1);
+                if (after - before == 2) moves.push_back((uint32_t)a | ((uint32_t)b << 8) | ((uint32_t)c << 16));
+            }
+    F.move_count = (int)moves.size();
+    F.fact[0] = 1;
+    for (int i = 1; i < 24; ++i) F.fact[i] = F.fact[i - 1] * (uint64_t)(i < 21 ? i : 1);
+    const uint64_t total = F.fact[n - 1];
+    F.rank32 = total < (1ULL << 32) ? 1 : 0;
+
+    int device = 0;
+    HIP_CHECK(hipGetDevice(&device));
+    std::lock_guard<std::mutex> lk(g_filter_mutex);
+    if ((size_t)device >= g_filter_ws.size()) g_filter_ws.resize(device + 1);
+    if (!g_filter_ws[device]) {
+        auto w = std::make_unique<FilterWorkspace>();
+        w->device = device;
+        HIP_CHECK(hipStreamCreateWithFlags(&w->stream, hipStreamNonBlocking));
+        HIP_CHECK(hipEventCreate(&w->ev0));
+        HIP_CHECK(hipEventCreate(&w->ev1));
+        w->total = make_dev(8, "filter.total");
+        g_filter_ws[device] = std::move(w);
+    }
+    FilterWorkspace &W = *g_filter_ws[device];
+    constexpr uint64_t kChunk = 1ULL << 32;   // ranks per launch chunk
+    const uint64_t chunk = std::min<uint64_t>(total, kChunk);
+    const size_t blocks = (size_t)((chunk + 255) / 256);
+    if (W.blocks < blocks) {
+        W.mask = make_dev(blocks * 4 * 8, "filter.mask");
+        W.counts = make_dev(blocks * 4, "filter.counts");
+        W.offsets = make_dev(blocks * 8, "filter.offsets");
+        W.blocks = blocks;
+    }
+    const uint64_t cap_rows = (uint64_t)std::min<int64_t>(out_cap, (int64_t)total);
+    const size_t out_bytes = std::max<size_t>((size_t)cap_rows * (size_t)n, 1);
+    if (W.out_bytes < out_bytes) {
+        W.out = make_dev(out_bytes, "filter.output");
+        W.out_bytes = out_bytes;
+    }
+    if (W.moves_cap < std::max<size_t>(moves.size(), 1)) {
+        W.moves = make_dev(std::max<size_t>(moves.size(), 1) * 4, "filter.moves");
+        W.moves_cap = std::max<size_t>(moves.size(), 1);
+    }
+    if (!moves.empty())
+        HIP_CHECK(hipMemcpyAsync(W.moves.get(), moves.data(), moves.size() * 4, hipMemcpyHostToDevice, W.stream));
+    F.moves = as<uint32_t>(W.moves);
+    F.mask = as<unsigned long long>(W.mask);
+    F.block_counts = as<uint32_t>(W.counts);
+    F.out = as<uint8_t>(W.out);
+    F.out_cap = cap_rows;
+    HIP_CHECK(hipMemsetAsync(W.total.get(), 0, 8, W.stream));
+    HIP_CHECK(hipEventRecord(W.ev0, W.stream));
+    for (uint64_t off = 0; off < total; off += chunk) {
+        F.rank_offset = off;
+        F.rank_end = std::min(total, off + chunk);
+        const uint64_t nb = (F.rank_end - off + 255) / 256;
+        hipLaunchKernelGGL(filter_mask_kernel, dim3((unsigned)nb), dim3(256), 0, W.stream, F);
+        HIP_CHECK(hipGetLastError());
+        hipLaunchKernelGGL(filter_scan_kernel, dim3(1), dim3(1024), 0, W.stream, as<uint32_t>(W.counts), nb,
+                           as<uint64_t>(W.offsets), as<uint64_t>(W.total));
+        HIP_CHECK(hipGetLastError());
+        if (cap_rows > 0) {
+            hipLaunchKernelGGL(filter_write_kernel, dim3((unsigned)nb), dim3(256), 0, W.stream, F,
+                               (const uint64_t *)as<uint64_t>(W.offsets));
+            HIP_CHECK(hipGetLastError());
+        }
+    }
+    HIP_CHECK(hipEventRecord(W.ev1, W.stream));
+    uint64_t count = 0;
+    HIP_CHECK(hipMemcpyAsync(&count, W.total.get(), 8, hipMemcpyDeviceToHost, W.stream));
+    HIP_CHECK(hipStreamSynchronize(W.stream));
+    const uint64_t rows = std::min<uint64_t>(count, cap_rows);
+    if (rows > 0)
+        HIP_CHECK(hipMemcpy(out, W.out.get(), (size_t)rows * (size_t)n, hipMemcpyDeviceToHost));
+    float ms = 0.f;
+    HIP_CHECK(hipEventElapsedTime(&ms, W.ev0, W.ev1));
+    g_filter_ms = ms;
+    if (out_total) *out_total = (int64_t)count;
+    if (out_written) *out_written = (int64_t)rows;
+}
+
+void host_eval_children(const uint8_t *pi, const uint8_t *spi, int n, int32_t *out) {
+    uint32_t info[64];
+    bool seen[64] = {false};
+    for (int s = 0; s < n; ++s) {
+        if (seen[s]) continue;
+        std::vector<int> cyc;
+        int x = s;
+        while (!seen[x]) {
+            seen[x] = true;
+            cyc.push_back(x);
+            x = spi[x];
+        }
+        const int L = (int)cyc.size();
+        for (int q = 0; q < L; ++q) info[cyc[q]] = cycle_info(cyc[0], q, L);
+    }
+    const int e = count_odd_cycles(spi, n);
+    int t = 0;
+    for (int i = 0; i < n - 2; ++i)
+        for (int j = i + 1; j < n - 1; ++j)
+            for (int k = j + 1; k < n; ++k) out[t++] = e + parity_delta(info[pi[i]], info[pi[j]], info[pi[k]]);
+}
+
+}  // namespace
+}  // namespace tdp
+
+using namespace tdp;
+
+extern "C" {
+
+const char *tdp_last_error(void) { return g_last_error.c_str(); }
+
+int tdp_device_count(void) {
+    int c = 0;
+    if (hipGetDeviceCount(&c) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    return c;
+}
+
+int tdp_init_slots(int total_slots, int devices_count, int64_t queue_bytes_budget, int dedup_table_size) {
+    return guarded([&] { init_slots(total_slots, devices_count, queue_bytes_budget, dedup_table_size); });
+}
+
+int tdp_destroy_slots(void) {
+    return guarded([&] { destroy_all_slots(); });
+}
+
+int tdp_search(int slot, const uint8_t *pi, const uint8_t *spi, int n, int initial_even_cycles, float min_rate,
+               int max_moves, int full_sorting, int32_t *out_moves, int32_t *out_depth) {
+    return search_impl(slot, pi, spi, n, initial_even_cycles, min_rate, max_moves, full_sorting, 0u, out_moves,
+                       out_depth, nullptr);
+}
+
+int tdp_search_ex(int slot, const uint8_t *pi, const uint8_t *spi, int n, int initial_even_cycles, float min_rate,
+                  int max_moves, int full_sorting, uint32_t flags, int32_t *out_moves, int32_t *out_depth,
+                  tdp_counters *out_counters) {
+    return search_impl(slot, pi, spi, n, initial_even_cycles, min_rate, max_moves, full_sorting, flags, out_moves,
+                       out_depth, out_counters);
+}
+
+int tdp_search_batch(int slot, int count, const uint8_t *pis, const uint8_t *spis, const int32_t *ns,
+                     const int32_t *e0s, const float *min_rates, const int32_t *max_moves,
+                     const int32_t *full_sorting, uint32_t flags, tdp_case_result *out) {
+    return guarded([&] {
+        if (count < 0) throw std::invalid_argument("negative batch size");
+        if (count == 0) return;
+        if (!pis || !spis || !ns || !e0s || !min_rates || !max_moves || !full_sorting || !out)
+            throw std::invalid_argument("null argument");
+        Slot &s = get_slot(slot);
+        std::lock_guard<std::mutex> run(s.run_mutex);
+        HIP_CHECK(hipSetDevice(s.device));
+        clear_cancel(s);
+        std::vector<CaseInput> cases;
+        std::vector<int> index;   // batch position of each launched case
+        std::memset(out, 0, sizeof(tdp_case_result) * (size_t)count);
+        for (int c = 0; c < count; ++c) {
+            CaseInput ci{pis + 64 * (size_t)c, spis + 64 * (size_t)c, ns[c], e0s[c], max_moves[c],
+                         full_sorting[c], min_rates[c]};
+            validate_case(ci);
+            if (triple_count(ci.n) == 0) continue;
+            cases.push_back(ci);
+            index.push_back(c);
+        }
+        if (cases.empty()) return;
+        run_launch(s, cases, flags, false);
+        const ConfigResult *res = as<ConfigResult>(s.h_results);
+        for (size_t q = 0; q < cases.size(); ++q) {
+            tdp_case_result &o = out[index[q]];
+            o.found = res[q].found ? 1 : 0;
+            o.depth = res[q].found ? res[q].depth : 0;
+            for (int m = 0; m < o.depth; ++m) triples().unpack(cases[q].n, res[q].moves[m], &o.moves[m][0]);
+            copy_counters(res[q], &o.counters);
+        }
+    });
+}
+
+int tdp_last_launch_stats(int slot, tdp_launch_stats *out) {
+    return guarded([&] {
+        if (!out) throw std::invalid_argument("null argument");
+        *out = get_slot(slot).stats;
+    });
+}
+
+void tdp_cancel(int slot) {
+    // slot_manager.cu:100-125: host flag, then an async copy of a pinned 1
+    // into the device flag on the slot's cancel stream.  Never throws.
+    try {
+        Slot &s = get_slot(slot);
+        s.cancelled.store(true, std::memory_order_release);
+        int cur = -1;
+        (void)hipGetDevice(&cur);
+        (void)hipSetDevice(s.device);
+        (void)hipMemcpyAsync(s.cancel_flag.get(), s.cancel_one.get(), sizeof(int), hipMemcpyHostToDevice,
+                             s.cancel_stream);
+        if (cur >= 0 && cur != s.device) (void)hipSetDevice(cur);
+    } catch (...) {
+    }
+}
+
+int tdp_filter(const uint8_t *cycle, int cycle_len, const uint8_t *triple, int triple_len, int n,
+               int skip_2move_check, uint8_t *out, int64_t out_rows_cap, int64_t *out_total, int64_t *out_written) {
+    return guarded([&] {
+        if (out_rows_cap > 0 && !out) throw std::invalid_argument("null argument");
+        filter_impl(cycle, cycle_len, triple, triple_len, n, skip_2move_check, out, out_rows_cap, out_total,
+                    out_written);
+    });
+}
+
+double tdp_filter_last_kernel_ms(void) { return g_filter_ms; }
+
+int tdp_eval_children(const uint8_t *pi, const uint8_t *spi, int n, int32_t *out) {
+    return guarded([&] {
+        if (n < 0 || n > kMaxN) throw std::invalid_argument("performGpuSortingSearch: n exceeds MAX_N");
+        const int T = triple_count(n);
+        if (T == 0) return;
+        CaseInput c{pi, spi, n, 0, 1, 0, 1.0f};
+        validate_case(c);
+        int dev = 0;
+        HIP_CHECK(hipGetDevice(&dev));
+        DeviceShared &ds = device_shared(dev);
+        DevPtr d_pi = make_dev(64, "eval.pi"), d_spi = make_dev(64, "eval.spi"), d_out = make_dev((size_t)T * 4, "eval.out");
+        HIP_CHECK(hipMemcpy(d_pi.get(), pi, (size_t)n, hipMemcpyHostToDevice));
+        HIP_CHECK(hipMemcpy(d_spi.get(), spi, (size_t)n, hipMemcpyHostToDevice));
+        const int e = count_odd_cycles(spi, n);
+        hipLaunchKernelGGL(eval_children_kernel, dim3(1), dim3(64), 0, nullptr, as<uint8_t>(d_pi), as<uint8_t>(d_spi),
+                           n, as<uint32_t>(ds.triples) + triples().offset[n], T, e, as<int32_t>(d_out));
+        HIP_CHECK(hipGetLastError());
+        HIP_CHECK(hipDeviceSynchronize());
+        HIP_CHECK(hipMemcpy(out, d_out.get(), (size_t)T * 4, hipMemcpyDeviceToHost));
+    });
+}
+
+int tdp_host_eval_children(const uint8_t *pi, const uint8_t *spi, int n, int32_t *out) {
+    return guarded([&] {
+        if (n < 0 || n > kMaxN) throw std::invalid_argument("performGpuSortingSearch: n exceeds MAX_N");
+        CaseInput c{pi, spi, n, 0, 1, 0, 1.0f};
+        validate_case(c);
+        host_eval_children(pi, spi, n, out);
+    });
+}
+
+int tdp_host_predicate_tables(int n, int e0, float min_rate, int max_depth, int full, uint64_t *acc,
+                              uint64_t *rch) {
+    return guarded([&] {
+        if (n < 0 || n > kMaxN) throw std::invalid_argument("performGpuSortingSearch: n exceeds MAX_N");
+        if (max_depth <= 0 || max_depth > kMaxDepth)
+            throw std::invalid_argument("performGpuSortingSearch: invalid maximumDepth");
+        predicate_tables(n, e0, min_rate, max_depth, full != 0, acc, rch);
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,33 @@
+// Launch interface of the filter kernels (tdp_filter.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace tdp {
+
+struct FilterParams {
+    int n;
+    int cycle_len;
+    int triple_len;
+    int move_count;          // 2-move candidate triples of the cycle
+    int skip_2move;
+    int rank32;              // (n-1)! < 2^32: 32-bit rank arithmetic
+    uint8_t cycle[64];
+    uint8_t triple[64];
+    const uint32_t *moves;   // packed a | b << 8 | c << 16 symbols
+    uint64_t fact[24];
+    uint64_t rank_offset;    // first rank of this launch
+    uint64_t rank_end;       // one past the last rank of this launch
+    unsigned long long *mask;
+    uint32_t *block_counts;
+    uint8_t *out;
+    uint64_t out_cap;
+};
+
+__global__ void filter_mask_kernel(FilterParams F);
+__global__ void filter_scan_kernel(const uint32_t *counts, uint64_t nblocks, uint64_t *offsets,
+                                   uint64_t *total);
+__global__ void filter_write_kernel(FilterParams F, const uint64_t *offsets);
+
+}  // namespace tdp

@@ -1,0 +1,64 @@
+"""GPU parity of the permutation filter: uncapped counts and rank-ordered rows."""
+import numpy as np
+import pytest
+
+import oracle
+import tdp1375_amd as T
+from tdp1375_amd.workloads import FILTER_ORIENTED_5_CYCLE
+
+pytestmark = pytest.mark.gpu
+
+CYCLE = FILTER_ORIENTED_5_CYCLE["cycle"]
+TRIPLE = FILTER_ORIENTED_5_CYCLE["triple"]
+
+
+@pytest.mark.parametrize("n", [1, 2, 5, 6, 8, 10])
+@pytest.mark.parametrize("skip", [False, True])
+def test_filter_small_exact(gpu_lib, n, skip):
+    cyc = [c for c in CYCLE if c < n] if n < 5 else CYCLE
+    tri = [t for t in TRIPLE if t < n]
+    rows, total = T.filter_pi_permutations(cyc, len(cyc), tri, len(tri), n, skip)
+    want_total, want_rows = oracle.filter_pi(cyc, tri, n, skip, 1 << 20, threads=8)
+    assert total == want_total
+    assert np.array_equal(rows, want_rows)
+
+
+@pytest.mark.parametrize("skip", [False, True])
+def test_filter_n11_all_ranks(gpu_lib, skip):
+    """n=11: all 10! arrangements."""
+    rows, total = T.filter_pi_permutations(CYCLE, 5, TRIPLE, 3, 11, skip)
+    want_total, want_rows = oracle.filter_pi(CYCLE, TRIPLE, 11, skip, 1 << 20, threads=16)
+    assert total == want_total
+    assert np.array_equal(rows, want_rows)
+
+
+def test_filter_n12_capped_rank_order(gpu_lib):
+    """n=12: all 11! arrangements; the JNI-sized list is the first 2^20 by rank."""
+    rows, total = T.filter_pi_permutations(CYCLE, 5, TRIPLE, 3, 12, False)
+    want_total, want_rows = oracle.filter_pi(CYCLE, TRIPLE, 12, False, 1 << 20, threads=16)
+    assert total == want_total
+    assert len(rows) == min(total, 1 << 20)
+    assert np.array_equal(rows, want_rows)
+
+
+def test_filter_other_cycles(gpu_lib):
+    """Longer cycles and triples, including a cycle with an even length."""
+    rng = np.random.default_rng(3)
+    for _ in range(6):
+        n = 9
+        sym = [int(x) for x in rng.permutation(n)]
+        L = int(rng.integers(3, 8))
+        cyc = sym[:L]
+        tri = sorted(sym[L:L + 3]) if L + 3 <= n else sorted(sym[:3])
+        for skip in (False, True):
+            rows, total = T.filter_pi_permutations(cyc, L, tri, len(tri), n, skip)
+            want_total, want_rows = oracle.filter_pi(cyc, tri, n, skip, 1 << 20, threads=8)
+            assert total == want_total
+            assert np.array_equal(rows, want_rows)
+
+
+def test_filter_invalid_arguments(gpu_lib):
+    with pytest.raises(T.TdpError, match="n must be at least 1"):
+        T.filter_pi_permutations(CYCLE, 5, TRIPLE, 3, 0, False)
+    with pytest.raises(T.TdpError, match="n exceeds PI_FILTER_MAX_N"):
+        T.filter_pi_permutations(CYCLE, 5, TRIPLE, 3, 64, False)

@@ -1,0 +1,91 @@
+"""CPU tests of the library's host side through the C ABI (no GPU compute)."""
+import ctypes
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle
+import tdp1375_amd as T
+from tdp1375_amd import _lib
+from tdp1375_amd import workloads as W
+from helpers import MIN_RATE_15, MIN_RATE_1375
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_symbols():
+    with open(os.path.join(ROOT, "include", "tdp1375.h")) as fh:
+        return re.findall(r"^TDP_API [^(]*?\b(tdp_\w+)\(", fh.read(), flags=re.M)
+
+
+def test_library_exports_every_declared_symbol():
+    lib = T.load()
+    declared = header_symbols()
+    assert len(declared) >= 14
+    for name in declared:
+        assert hasattr(lib, name), name
+    for name in _lib.EXPORTED_SYMBOLS:
+        assert hasattr(lib, name), name
+    out = subprocess.run(["nm", "-D", "--defined-only", _lib.LIB_PATH], capture_output=True, text=True).stdout
+    exported = set(re.findall(r" T (\S+)", out))
+    assert set(declared) <= exported
+    assert {s for s in _lib.EXPORTED_SYMBOLS if s.startswith("Java_")} <= exported
+
+
+def test_host_parity_delta_equals_recount():
+    lib = T.load()
+    rng = np.random.default_rng(11)
+    for it in range(80):
+        n = int(rng.integers(3, 64))
+        pi = [int(x) for x in rng.permutation(n)]
+        spi = [int(x) for x in rng.permutation(n)] if it % 2 else W.one_line(
+            [list(map(int, rng.permutation(n)[: int(rng.integers(2, 8))]))], n)
+        want = oracle.children_even(pi, spi)
+        got = np.zeros(len(want), np.int32)
+        _lib.check(lib.tdp_host_eval_children(_lib.u8(pi, 64), _lib.u8(spi, 64), n,
+                                              got.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))))
+        assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("rate", [MIN_RATE_1375, MIN_RATE_15, float(np.float32(2) / np.float32(1.25)), 1.0])
+def test_predicate_tables_equal_float32_predicates(rate):
+    lib = T.load()
+    acc = (ctypes.c_uint64 * 16)()
+    rch = (ctypes.c_uint64 * 16)()
+    for n in range(3, 64, 4):
+        for e0 in range(0, n + 1, 5):
+            for D in (1, 4, 9, 11, 15):
+                for full in (0, 1):
+                    _lib.check(lib.tdp_host_predicate_tables(n, e0, rate, D, full, acc, rch))
+                    for d in range(16):
+                        for e in range(n + 1):
+                            assert bool(acc[d] >> e & 1) == oracle.satisfies_rate(e, d, n, e0, rate, full)
+                            assert bool(rch[d] >> e & 1) == oracle.can_still_reach(e, d, n, e0, rate, D, full)
+
+
+def test_argument_errors_without_gpu():
+    """Validation happens before any device work, with the reference's messages."""
+    with pytest.raises(T.TdpError, match="performGpuSortingSearch: n exceeds MAX_N"):
+        T.search_for_sorting_seq(list(range(64)), list(range(64)), 0, 1.0, 4, False, 0)
+    with pytest.raises(T.TdpError, match="performGpuSortingSearch: invalid maximumDepth"):
+        T.search_for_sorting_seq(list(range(8)), list(range(8)), 8, 1.0, 16, False, 0)
+    with pytest.raises(T.TdpError, match="invalid slot"):
+        T.search_for_sorting_seq(list(range(8)), list(range(8)), 8, 1.0, 4, False, 123)
+    with pytest.raises(T.TdpError, match="n must be at least 1"):
+        T.filter_pi_permutations([0], 1, [0], 1, 0, False)
+    with pytest.raises(T.TdpError, match="n exceeds PI_FILTER_MAX_N"):
+        T.filter_pi_permutations([0], 1, [0], 1, 64, False)
+    T.cancel_sort(-5)   # never throws (slot_manager.cu:100-125)
+
+
+def test_positions_to_symbols_replay():
+    """GPUSortingSearch.java:65-87: positions are replayed on a copy of pi."""
+    pi = [0, 1, 2, 3, 4, 5]
+    moves = T.positions_to_symbols(pi, [(0, 2, 4), (1, 3, 5)])
+    assert moves[0] == (0, 2, 4)
+    # after the first block swap pi = [2, 3, 0, 1, 4, 5]
+    assert moves[1] == (3, 1, 5)
+    assert pi == [0, 1, 2, 3, 4, 5]

@@ -1,0 +1,184 @@
+"""The JNI natives of GPUSortingSearch driven through a fake JNIEnv.
+
+The fake environment is a function table (JNI specification indices) whose
+entries are ctypes callbacks keeping Java objects as Python values, so the
+shim's marshalling and exception mapping (jni_sorter.cu:16-150) are checked
+without a JVM.  GPU round trips are marked gpu.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+import oracle
+import tdp1375_amd as T
+from tdp1375_amd import _lib
+from tdp1375_amd import workloads as W
+
+VP = ctypes.c_void_p
+JINT = ctypes.c_int32
+
+
+class FakeJNIEnv:
+    IDX = dict(FindClass=6, ThrowNew=14, ExceptionClear=17, DeleteLocalRef=23, GetArrayLength=171,
+               NewObjectArray=172, SetObjectArrayElement=174, NewByteArray=176, NewIntArray=179,
+               GetByteArrayRegion=200, SetByteArrayRegion=208, SetIntArrayRegion=211, ExceptionCheck=228)
+
+    def __init__(self):
+        self.objects = {}
+        self.next_handle = 16
+        self.exception = None
+        self.live_refs = 0
+        self._keep = []
+        self.table = (VP * 235)()
+        cb = self._install
+        cb("FindClass", ctypes.CFUNCTYPE(VP, VP, ctypes.c_char_p), lambda env, name: self._new(("class", name.decode())))
+        cb("ThrowNew", ctypes.CFUNCTYPE(JINT, VP, VP, ctypes.c_char_p), self._throw_new)
+        cb("ExceptionClear", ctypes.CFUNCTYPE(None, VP), lambda env: setattr(self, "exception", None))
+        cb("ExceptionCheck", ctypes.CFUNCTYPE(ctypes.c_uint8, VP), lambda env: int(self.exception is not None))
+        cb("DeleteLocalRef", ctypes.CFUNCTYPE(None, VP, VP), self._delete)
+        cb("GetArrayLength", ctypes.CFUNCTYPE(JINT, VP, VP), lambda env, a: len(self.objects[a][1]))
+        cb("NewObjectArray", ctypes.CFUNCTYPE(VP, VP, JINT, VP, VP),
+           lambda env, n, cls, init: self._new(("objarray", [None] * n)))
+        cb("SetObjectArrayElement", ctypes.CFUNCTYPE(None, VP, VP, JINT, VP), self._set_elem)
+        cb("NewByteArray", ctypes.CFUNCTYPE(VP, VP, JINT), lambda env, n: self._new(("bytes", [0] * n)))
+        cb("NewIntArray", ctypes.CFUNCTYPE(VP, VP, JINT), lambda env, n: self._new(("ints", [0] * n)))
+        cb("GetByteArrayRegion", ctypes.CFUNCTYPE(None, VP, VP, JINT, JINT, ctypes.POINTER(ctypes.c_int8)),
+           self._get_bytes)
+        cb("SetByteArrayRegion", ctypes.CFUNCTYPE(None, VP, VP, JINT, JINT, ctypes.POINTER(ctypes.c_int8)),
+           self._set_region)
+        cb("SetIntArrayRegion", ctypes.CFUNCTYPE(None, VP, VP, JINT, JINT, ctypes.POINTER(JINT)),
+           self._set_region)
+        self.env_struct = (VP * 1)(ctypes.cast(self.table, VP))
+        self.env = ctypes.cast(self.env_struct, VP)
+
+    def _install(self, name, proto, fn):
+        f = proto(fn)
+        self._keep.append(f)
+        self.table[self.IDX[name]] = ctypes.cast(f, VP)
+
+    def _new(self, obj):
+        h = self.next_handle
+        self.next_handle += 8
+        self.objects[h] = obj
+        self.live_refs += 1
+        return h
+
+    def _delete(self, env, h):
+        self.live_refs -= 1
+
+    def _throw_new(self, env, cls, msg):
+        self.exception = (self.objects[cls][1], msg.decode())
+        return 0
+
+    def _set_elem(self, env, arr, i, v):
+        self.objects[arr][1][i] = v
+
+    def _get_bytes(self, env, arr, start, n, buf):
+        data = self.objects[arr][1]
+        for q in range(n):
+            buf[q] = ctypes.c_int8(data[start + q]).value
+
+    def _set_region(self, env, arr, start, n, buf):
+        data = self.objects[arr][1]
+        for q in range(n):
+            data[start + q] = int(buf[q])
+
+    def byte_array(self, values):
+        return self._new(("bytes", [ctypes.c_int8(v & 0xFF).value for v in values]))
+
+    def value(self, h):
+        kind, data = self.objects[h]
+        if kind == "objarray":
+            return [self.value(x) for x in data]
+        if kind == "bytes":
+            return [v & 0xFF for v in data]
+        return list(data)
+
+
+def natives():
+    lib = T.load()
+    pre = "Java_br_unb_cic_tdp_base_GPUSortingSearch_"
+    init = getattr(lib, pre + "initSlots")
+    init.argtypes = [VP, VP, JINT, JINT, ctypes.c_int64, JINT]
+    init.restype = None
+    destroy = getattr(lib, pre + "destroySlots")
+    destroy.argtypes = [VP, VP]
+    destroy.restype = None
+    search = getattr(lib, pre + "searchForSortingSeq")
+    search.argtypes = [VP, VP, VP, VP, JINT, ctypes.c_float, JINT, ctypes.c_uint8, JINT]
+    search.restype = VP
+    cancel = getattr(lib, pre + "cancelSort")
+    cancel.argtypes = [VP, VP, JINT]
+    cancel.restype = None
+    filt = getattr(lib, pre + "filterPiPermutations")
+    filt.argtypes = [VP, VP, VP, JINT, VP, JINT, JINT, ctypes.c_uint8]
+    filt.restype = VP
+    return init, destroy, search, cancel, filt
+
+
+def test_errors_become_runtime_exceptions():
+    init, destroy, search, cancel, filt = natives()
+    env = FakeJNIEnv()
+    # filterPiPermutations with n = 0 (pi_filter.cu:166)
+    r = filt(env.env, None, env.byte_array([0, 3, 1, 4, 2]), 5, env.byte_array([0, 1, 2]), 3, 0, 0)
+    assert r is None
+    assert env.exception == ("java/lang/RuntimeException", "n must be at least 1")
+    # searchForSortingSeq on a slot that does not exist
+    env.exception = None
+    pi = list(range(8))
+    r = search(env.env, None, env.byte_array(pi), env.byte_array(pi), 8, 1.0, 4, 0, 99)
+    assert r is None
+    assert env.exception == ("java/lang/RuntimeException", "invalid slot")
+    # spi shorter than pi
+    env.exception = None
+    r = search(env.env, None, env.byte_array(pi), env.byte_array(pi[:4]), 8, 1.0, 4, 0, 0)
+    assert r is None and "inconsistent input sizes" in env.exception[1]
+    # invalid maximumDepth (sorting_search.cu:715)
+    env.exception = None
+    r = search(env.env, None, env.byte_array(pi), env.byte_array(pi), 8, 1.0, 16, 0, 0)
+    assert r is None and env.exception[1] == "performGpuSortingSearch: invalid maximumDepth"
+    # initSlots with no slots
+    env.exception = None
+    init(env.env, None, 0, 1, 1 << 20, 1 << 10)
+    assert env.exception == ("java/lang/RuntimeException", "totalSlots must be positive")
+    # cancelSort never throws
+    env.exception = None
+    cancel(env.env, None, 3)
+    assert env.exception is None
+
+
+def test_bytes_are_read_unsigned():
+    """Symbols >= 128 must arrive as n >= 128 and be rejected as n-range errors,
+    not as negative values (jni_sorter.cu:63,68 mask with 0xFF)."""
+    init, destroy, search, cancel, filt = natives()
+    env = FakeJNIEnv()
+    r = filt(env.env, None, env.byte_array([200]), 1, env.byte_array([0]), 1, 5, 1)
+    assert r is None and env.exception[1] == "cycle symbol out of range"
+
+
+@pytest.mark.gpu
+def test_jni_round_trip_on_gpu(gpu_lib):
+    init, destroy, search, cancel, filt = natives()
+    env = FakeJNIEnv()
+    init(env.env, None, 1, 1, 64 << 20, 1 << 20)
+    assert env.exception is None
+    try:
+        wl = W.workload_random_n12()
+        for case in wl.cases[:8]:
+            r = search(env.env, None, env.byte_array(case.pi), env.byte_array(case.spi), case.even_cycles,
+                       wl.min_rate, wl.max_depth, 0, 0)
+            assert env.exception is None and r is not None
+            positions = [tuple(m) for m in env.value(r)]
+            want = oracle.search(case.pi, case.spi, case.even_cycles, wl.min_rate, wl.max_depth)
+            assert bool(positions) == want.found
+            if positions:
+                e, _, _ = oracle.replay_positions(case.pi, case.spi, positions)
+                assert oracle.satisfies_rate(e, len(positions), case.n, case.even_cycles, wl.min_rate, False)
+        r = filt(env.env, None, env.byte_array([0, 3, 1, 4, 2]), 5, env.byte_array([0, 1, 2]), 3, 8, 0)
+        rows = env.value(r)
+        _, want_rows = oracle.filter_pi([0, 3, 1, 4, 2], [0, 1, 2], 8, False, 1 << 20)
+        assert np.array_equal(np.array(rows, np.uint8).reshape(-1, 8), want_rows)
+    finally:
+        destroy(env.env, None)
+    assert env.exception is None

@@ -1,0 +1,278 @@
+"""Pins the CPU oracle against the reference's known answers, its case files,
+closed forms and brute force (no GPU)."""
+import itertools
+
+import numpy as np
+import pytest
+
+import oracle
+from tdp1375_amd import workloads as W
+from helpers import MIN_RATE_15, MIN_RATE_1375, case_inputs, load_cases
+
+
+# ---------------------------------------------------------------- conventions
+
+def test_odd_cycle_count_closed_forms():
+    for n in range(1, 64):
+        assert oracle.count_odd_cycles(list(range(n))) == n           # identity: n fixed points
+        assert oracle.count_odd_cycles([(i + 1) % n for i in range(n)]) == (n & 1)  # one n-cycle
+    # Cycle.isEven() is "odd length" (Cycle.java:149-151)
+    assert oracle.count_odd_cycles(W.one_line([(0, 4, 2), (1, 5, 3)], 6)) == 2
+    assert oracle.count_odd_cycles(W.one_line([(0, 1), (2, 3)], 4)) == 0
+
+
+def test_block_swap_known_answer():
+    """CommonOperationsTest.java:21-24: (0 4 5 6 1 2 3) . (0 4 1) -> (0 1 2 3 4 5 6)."""
+    pi = [0, 4, 5, 6, 1, 2, 3]
+    spi = list(range(7))
+    # symbols 0, 4, 1 sit at positions 0, 1, 4
+    p2, _ = oracle.apply_move(pi, spi, 0, 1, 4)
+    rot = p2.index(0)
+    assert p2[rot:] + p2[:rot] == list(range(7))
+
+
+def test_product_known_answer():
+    """PermutationGroupsTest.java:9-12: (0 4 1)(0 4 5 6 1 2 3) = (0 1 2 3 4 5 6), i.e.
+    the cyclic pi after the move is tau o pi."""
+    pi = [0, 4, 5, 6, 1, 2, 3]
+    tau = {0: 4, 4: 1, 1: 0}
+    succ = {pi[t]: pi[(t + 1) % 7] for t in range(7)}
+    prod = {x: tau.get(succ[x], succ[x]) for x in range(7)}
+    assert all(prod[x] == (x + 1) % 7 for x in range(7))
+    p2, _ = oracle.apply_move(pi, list(range(7)), 0, 1, 4)
+    succ2 = {p2[t]: p2[(t + 1) % 7] for t in range(7)}
+    assert succ2 == prod
+
+
+def test_spi_update_is_right_multiplication_by_inverse():
+    """spi' = spi o (a b c)^-1 (sorting_search.cu:551-555) on random inputs."""
+    rng = np.random.default_rng(1)
+    for _ in range(200):
+        n = int(rng.integers(3, 20))
+        pi = [int(x) for x in rng.permutation(n)]
+        spi = [int(x) for x in rng.permutation(n)]
+        i, j, k = sorted(int(x) for x in rng.choice(n, 3, replace=False))
+        a, b, c = pi[i], pi[j], pi[k]
+        inv = {a: c, b: a, c: b}
+        want = [spi[inv.get(x, x)] for x in range(n)]
+        _, s2 = oracle.apply_move(pi, spi, i, j, k)
+        assert s2 == want
+
+
+# ---------------------------------------------------------------- predicates
+
+def _accept_int(e, d, n, e0, full):
+    g = e - e0
+    if d <= 0 or (full and e != n):
+        return False
+    return g > 0 and 11 * g >= 16 * d
+
+
+def _reach_int(e, d, n, e0, D, full):
+    g = e - e0
+    if D - d < 0:
+        return False
+    M = n - e
+    half = (M + 1) // 2
+    if full:
+        if d + half > D:
+            return False
+        return g + 2 * (D - d) >= -(-16 * (d + half) // 11)
+    if g + 2 * (D - d) < (16 * D) // 11:
+        return False
+    if d + half > 0 and 11 * (g + M) < 16 * (d + half):
+        return False
+    return True
+
+
+def test_predicates_equal_integer_closed_forms_at_1375():
+    """The reference's float32 predicates at max_ratio 1.375 equal integer forms
+    (accept <=> g > 0 and 11g >= 16d, ...) on the whole domain n <= 63, d <= 15."""
+    mism = 0
+    for n in range(1, 64, 2):
+        for e0 in range(0, n + 1, 3):
+            for D in (1, 4, 8, 9, 11, 15):
+                for full in (False, True):
+                    for d in range(0, 16):
+                        for e in range(0, n + 1):
+                            a = oracle.satisfies_rate(e, d, n, e0, MIN_RATE_1375, full)
+                            r = oracle.can_still_reach(e, d, n, e0, MIN_RATE_1375, D, full)
+                            mism += a != _accept_int(e, d, n, e0, full)
+                            mism += r != _reach_int(e, d, n, e0, D, full)
+    assert mism == 0
+
+
+# ---------------------------------------------------------------- search
+
+def _small_cases(count, seed, nlo=5, nhi=8):
+    rng = W.SplitMix64(seed)
+    out = []
+    for _ in range(count):
+        n = nlo + rng.below(nhi - nlo + 1)
+        pi = [0] + rng.shuffle(list(range(1, n)))
+        out.append(W.Case(pi, W.spi_from_pi(pi)))
+    return out
+
+
+@pytest.mark.parametrize("rate,D", [(MIN_RATE_1375, 4), (MIN_RATE_1375, 6), (MIN_RATE_15, 3), (MIN_RATE_15, 5)])
+def test_search_verdict_equals_brute_force(rate, D):
+    for c in _small_cases(60, 17):
+        r = oracle.search(c.pi, c.spi, c.even_cycles, rate, D)
+        assert r.found == oracle.bfs_verdict(c.pi, c.spi, c.even_cycles, rate, D)
+        assert r.found == oracle.search(c.pi, c.spi, c.even_cycles, rate, D, use_dedup=False).found
+        if r.found:
+            e, _, _ = oracle.replay_positions(c.pi, c.spi, r.moves)
+            assert oracle.satisfies_rate(e, r.depth, c.n, c.even_cycles, rate, False)
+
+
+@pytest.mark.parametrize("D", [3, 5, 7])
+def test_full_sort_verdicts_agree(D):
+    """fullSorting: search == brute force == SortingSearch.java restatement with the
+    kernel's float32 prune; the Java arithmetic (double product in the prune bound,
+    SortingSearch.java:63) only ever prunes more, at exact-threshold ratios."""
+    stricter = 0
+    for c in _small_cases(40, 23, 5, 8):
+        r = oracle.search(c.pi, c.spi, c.even_cycles, MIN_RATE_15, D, full=True)
+        bfs = oracle.bfs_verdict(c.pi, c.spi, c.even_cycles, MIN_RATE_15, D, full=True)
+        java_f32 = oracle.java_full_sort_search(c.pi, c.spi, c.even_cycles, MIN_RATE_15, D, float_prune=True)
+        java = oracle.java_full_sort_search(c.pi, c.spi, c.even_cycles, MIN_RATE_15, D)
+        assert r.found == bfs == bool(java_f32)
+        assert not java or r.found
+        stricter += r.found and not java
+        for seq in (java, java_f32):
+            if seq:
+                e, _, _ = oracle.replay_symbols(c.pi, c.spi, seq)
+                assert e == c.n
+
+
+def test_java_double_prune_rejects_exact_threshold():
+    """Known divergence of the reference's two paths: 3 moves gaining 4 is exactly
+    ratio 1.5; the Java double bound ceil(1.3333334 * 3.0) = 5 prunes it, the
+    kernel's float32 bound ceil(4.0f) = 4 keeps it."""
+    pi = [0, 6, 5, 3, 4, 1, 2]
+    spi = W.spi_from_pi(pi)
+    e0 = oracle.count_odd_cycles(spi)
+    assert oracle.search(pi, spi, e0, MIN_RATE_15, 3, full=True).found
+    assert oracle.bfs_verdict(pi, spi, e0, MIN_RATE_15, 3, full=True)
+    assert oracle.java_full_sort_search(pi, spi, e0, MIN_RATE_15, 3, float_prune=True)
+    assert not oracle.java_full_sort_search(pi, spi, e0, MIN_RATE_15, 3)
+
+
+def test_case_file_witnesses_replay():
+    """All 690 witnesses of the reference's case files replay with the claimed gains."""
+    data = load_cases()
+    classes = {"comb": {(4, 6), (8, 12), (11, 16)}, "3_2": {(3, 4)}, "oriented_7cycle": {(4, 6)}}
+    total = 0
+    for key, entries in data.items():
+        for ent in entries:
+            pi, spi = case_inputs(ent)
+            e0 = oracle.count_odd_cycles(spi)
+            e, _, _ = oracle.replay_symbols(pi, spi, ent["witness"])
+            assert e >= 0
+            assert (len(ent["witness"]), e - e0) in classes[key]
+            total += 1
+    assert total == 690
+
+
+def test_case_files_found_by_oracle():
+    data = load_cases()
+    for ent in data["3_2"] + data["oriented_7cycle"]:
+        pi, spi = case_inputs(ent)
+        e0 = oracle.count_odd_cycles(spi)
+        r = oracle.search(pi, spi, e0, MIN_RATE_15, len(ent["witness"]))
+        assert r.found
+        if len(pi) <= 9:
+            assert oracle.bfs_verdict(pi, spi, e0, MIN_RATE_15, len(ent["witness"]))
+    # (4,6) entries of cases-comb are found at D=4 at ratio 1.375
+    for ent in [e for e in data["comb"] if len(e["witness"]) == 4][:10]:
+        pi, spi = case_inputs(ent)
+        r = oracle.search(pi, spi, oracle.count_odd_cycles(spi), MIN_RATE_1375, 4)
+        assert r.found
+
+
+def test_bad_small_components_not_found_by_oracle():
+    """Combinations.java:31-46 components admit no (11/8)-sequence within 11 moves."""
+    for c in W.bad_small_components():
+        r = oracle.search(c.pi, c.spi, c.even_cycles, MIN_RATE_1375, 11)
+        assert not r.found
+        if c.n <= 9:
+            assert not oracle.bfs_verdict(c.pi, c.spi, c.even_cycles, MIN_RATE_1375, 11)
+
+
+# ---------------------------------------------------------------- filter
+
+def _java_cyclic_order(cycle_symbols, symbols):
+    """CommonOperations.areSymbolsInCyclicOrder (CommonOperations.java:167-182)."""
+    idx = {s: i for i, s in enumerate(cycle_symbols)}
+    leap = False
+    for i in range(len(symbols)):
+        if idx[symbols[i]] > idx[symbols[(i + 1) % len(symbols)]]:
+            if not leap:
+                leap = True
+            else:
+                return False
+    return True
+
+
+def _java_is_oriented(pi_cycle, cycle):
+    """CommonOperations.isOriented (CommonOperations.java:443-445); Cycle.getInverse
+    reverses the symbol array (Cycle.java:86-94)."""
+    return not _java_cyclic_order(list(reversed(pi_cycle)), cycle)
+
+
+def _java_has_2move(pi_cycle, cycle):
+    """generateAll2MovesFromOrientedCycles for one cycle (CommonOperations.java:406-431)."""
+    L = len(cycle)
+    idx = {s: i for i, s in enumerate(cycle)}
+
+    def getk(a, b):
+        ai, bi = idx[a], idx[b]
+        return bi - ai if bi >= ai else (L - ai) + bi
+
+    before = 1 if L % 2 == 1 else 0
+    for i, j, k in itertools.combinations(range(L), 3):
+        a, b, c = cycle[i], cycle[j], cycle[k]
+        if _java_cyclic_order(pi_cycle, [a, b, c]):
+            after = (getk(a, b) % 2 == 1) + (getk(b, c) % 2 == 1) + (getk(c, a) % 2 == 1)
+            if after - before == 2:
+                return True
+    return False
+
+
+def test_java_predicate_known_answers():
+    """CommonOperationsTest.java:36-46."""
+    alpha = [0, 2, 7]
+    pi = [0, 5, 4, 3, 8, 7, 6, 2, 1]
+    assert _java_cyclic_order(list(reversed(pi)), alpha)
+    assert not _java_cyclic_order(pi, alpha)
+    assert _java_is_oriented(pi, list(reversed(alpha)))
+    assert not _java_is_oriented(pi, alpha)
+
+
+@pytest.mark.parametrize("n,cycle,triple", [
+    (5, [0, 3, 1, 4, 2], [0, 1, 2]),
+    (7, [0, 3, 1, 4, 2], [0, 1, 2]),
+    (8, [0, 3, 1, 4, 2], [0, 1, 2]),
+    (8, [2, 6, 1, 7], [0, 3, 5]),
+    (8, [1, 5, 2, 7, 4, 6], [0, 2, 4]),
+])
+@pytest.mark.parametrize("skip", [False, True])
+def test_filter_matches_java_predicates(n, cycle, triple, skip):
+    """pi_filter.cu enumeration == the Java predicates on every arrangement."""
+    total, rows = oracle.filter_pi(cycle, triple, n, skip, 1 << 16)
+    want = []
+    for rest in itertools.permutations(range(1, n)):
+        pi = [0] + list(rest)
+        ok = (_java_cyclic_order(pi, triple) and _java_is_oriented(pi, cycle)
+              and (skip or not _java_has_2move(pi, cycle)))
+        if ok:
+            want.append(pi)
+    assert total == len(want)
+    assert [list(r) for r in rows] == want   # itertools order == Lehmer rank order
+
+
+def test_filter_oriented_5_cycle_counts_by_symmetry():
+    """With skip, the triple/orientation predicates split the 24 arrangements of n=5."""
+    total_skip, _ = oracle.filter_pi([0, 3, 1, 4, 2], [0, 1, 2], 5, True, 100)
+    total, _ = oracle.filter_pi([0, 3, 1, 4, 2], [0, 1, 2], 5, False, 100)
+    assert 0 < total <= total_skip <= 24
