@@ -67,12 +67,28 @@ typedef struct tdp_launch_stats {
     uint64_t donations;      /* nodes moved to the shared pool               */
     uint64_t steals;         /* nodes taken from the shared pool             */
     uint64_t node_bytes;     /* bytes per node                               */
+    uint64_t pops;           /* nodes read from a wave's stack               */
+    uint64_t pushes;         /* nodes written to a wave's stack              */
+    uint64_t probes;         /* dedup tag words read                         */
+    uint64_t key_reads;      /* dedup keys compared                          */
+    uint64_t key_writes;     /* dedup entries written                        */
+    uint64_t key_bytes;      /* bytes of one stored key                      */
+    uint64_t overflow_info;  /* on overflow: reason << 56 (1 no sharing, 2 pool */
+                             /* full, 3 stack still full) | stack fill << 32 |  */
+                             /* pool fill                                      */
+    uint64_t phase_ticks[8]; /* profiling builds only: 10 ns ticks summed over */
+                             /* waves for pop, checks, cycle analysis, scan,  */
+                             /* dedup, push, sharing, root expansion          */
 } tdp_launch_stats;
 
 /* search flags */
 #define TDP_FLAG_DETERMINISTIC 1u   /* one wave per configuration, no sharing: */
                                     /* counters are reproducible                */
 #define TDP_FLAG_SINGLE_WAVE   2u   /* the whole launch runs on one wave        */
+#define TDP_FLAG_RESIDENT      4u   /* tdp_search_batch: reuse the configurations */
+                                    /* already in device memory from the previous */
+                                    /* batch on this slot (same count); the host  */
+                                    /* arrays are not read or uploaded            */
 
 TDP_API const char *tdp_last_error(void);
 

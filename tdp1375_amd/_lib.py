@@ -10,7 +10,8 @@ import os
 import threading
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libtdp1375_jni.so")
+# TDP_LIB_PATH selects another build of the same library (e.g. a profiling build)
+LIB_PATH = os.environ.get("TDP_LIB_PATH") or os.path.join(HERE, "libtdp1375_jni.so")
 MAX_DEPTH = 15
 FILTER_MAX_OUTPUT = 1 << 20
 
@@ -24,6 +25,7 @@ TDP_ERR_INTERNAL = 6
 
 FLAG_DETERMINISTIC = 1
 FLAG_SINGLE_WAVE = 2
+FLAG_RESIDENT = 4
 
 EXPORTED_SYMBOLS = (
     "tdp_last_error", "tdp_init_slots", "tdp_destroy_slots", "tdp_search", "tdp_search_ex",
@@ -54,7 +56,12 @@ class CaseResult(ctypes.Structure):
 
 class LaunchStats(ctypes.Structure):
     _fields_ = [("kernel_ms", ctypes.c_double), ("waves", ctypes.c_int32), ("overflowed", ctypes.c_int32),
-                ("donations", ctypes.c_uint64), ("steals", ctypes.c_uint64), ("node_bytes", ctypes.c_uint64)]
+                ("donations", ctypes.c_uint64), ("steals", ctypes.c_uint64), ("node_bytes", ctypes.c_uint64),
+                ("pops", ctypes.c_uint64), ("pushes", ctypes.c_uint64), ("probes", ctypes.c_uint64),
+                ("key_reads", ctypes.c_uint64), ("key_writes", ctypes.c_uint64), ("key_bytes", ctypes.c_uint64),
+                ("overflow_info", ctypes.c_uint64), ("phase_ticks", ctypes.c_uint64 * 8)]
+
+PHASES = ("pop", "check", "analyse", "scan", "dedup", "push", "share", "root")
 
 
 class TdpError(RuntimeError):

@@ -36,34 +36,43 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found")
 
 
-def _compile(src: str) -> str:
-    obj = os.path.join(OUT_DIR, os.path.splitext(src)[0] + ".o")
+def _compile(src: str, out_dir: str = OUT_DIR, defines=()) -> str:
+    obj = os.path.join(out_dir, os.path.splitext(src)[0] + ".o")
     path = os.path.join(CSRC, src)
+    extra = [f"-D{d}" for d in defines]
     if src.endswith(".hip"):
-        cmd = [hipcc()] + COMMON + HIP_FLAGS + ["-x", "hip", "-c", path, "-o", obj]
+        cmd = [hipcc()] + COMMON + HIP_FLAGS + extra + ["-x", "hip", "-c", path, "-o", obj]
     else:
-        cmd = [hipcc()] + COMMON + ["-c", path, "-o", obj]
+        cmd = [hipcc()] + COMMON + extra + ["-c", path, "-o", obj]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError(f"compile failed: {' '.join(cmd)}\n{res.stdout}\n{res.stderr}")
     return obj
 
 
-def build(verbose: bool = False) -> str:
-    os.makedirs(OUT_DIR, exist_ok=True)
+def build(verbose: bool = False, out: str = LIB, defines=()) -> str:
+    """Compiles and links the library.  `defines` adds -D macros (e.g.
+    TDP_PHASE_TIMING for a profiling build) and `out` names the .so."""
+    out_dir = OUT_DIR if not defines else OUT_DIR + "_" + "_".join(d.lower() for d in defines)
+    os.makedirs(out_dir, exist_ok=True)
     with cf.ThreadPoolExecutor(max_workers=len(SOURCES)) as ex:
-        objs = list(ex.map(_compile, SOURCES))
-    tmp = LIB + ".tmp"
+        objs = list(ex.map(lambda s: _compile(s, out_dir, defines), SOURCES))
+    tmp = out + ".tmp"
     cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs + ["-lpthread"]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{res.stdout}\n{res.stderr}")
-    os.replace(tmp, LIB)
+    os.replace(tmp, out)
     if verbose:
-        print("built", LIB)
-    return LIB
+        print("built", out)
+    return out
 
 
 if __name__ == "__main__":
-    build(verbose=True)
+    import argparse
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--phase-timing", action="store_true", help="profiling build with per-phase timers")
+    ap.add_argument("--out", default=LIB)
+    a = ap.parse_args()
+    build(verbose=True, out=a.out, defines=("TDP_PHASE_TIMING",) if a.phase_timing else ())
     sys.exit(0)

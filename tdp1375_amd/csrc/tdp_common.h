@@ -86,7 +86,7 @@ struct alignas(16) ConfigDesc {
     uint32_t max_depth;
     uint32_t e_start;        // odd-cycle count of spi0 (start node)
     uint32_t triple_count;
-    uint32_t triple_offset;  // offset of this n's triples in the triple table
+    uint32_t reserved0;
     uint32_t flags;          // bit 0: full sorting
     uint32_t pad[2];
     uint64_t accept[16];     // bit e of accept[d]: satisfies_rate(e, d)
@@ -138,12 +138,32 @@ enum ControlIndex {
     kCtlRegistered,
     kCtlPoolHead,
     kCtlPoolTail,
-    kCtlTakes,
+    kCtlReserved5,     // unused
     kCtlStop,          // 1 overflow
     kCtlResolved,      // configurations found
     kCtlDonations,
     kCtlSteals,
-    kNumControl
+    kCtlPops,          // nodes read from a stack (including discarded ones)
+    kCtlPushes,        // nodes written to a stack
+    kCtlProbes,        // dedup tag words read
+    kCtlKeyReads,      // dedup entries whose stored key was compared
+    kCtlKeyWrites,     // dedup entries written
+    kCtlOverflowInfo,  // why the overflow flag was raised: reason << 56 | pool fill
+    kCtlPhase0,        // per-phase time of profiling builds (kNumPhases lines)
+    kNumControl = kCtlPhase0 + 8
+};
+
+// Phases timed by profiling builds of the search kernel.
+enum PhaseIndex {
+    kPhasePop = 0,      // stack pop into LDS
+    kPhaseCheck,        // stop / found checks and config switch
+    kPhaseAnalyse,      // cycle analysis of the node
+    kPhaseScan,         // child evaluation over all triples
+    kPhaseDedup,        // dedup of staged children
+    kPhasePush,         // pushes of admitted children
+    kPhaseShare,        // donation, idle polling and pool takes
+    kPhaseRoot,         // root expansions
+    kNumPhases
 };
 
 }  // namespace tdp

@@ -257,7 +257,7 @@ const TripleTable &triples() {
 }
 
 struct DeviceShared {
-    DevPtr triples;
+
     int waves_max = 0;   // resident waves of search_kernel
     int cus = 0;
 };
@@ -269,9 +269,9 @@ DeviceShared &device_shared(int device) {
     if ((size_t)device >= g_dev.size()) g_dev.resize(device + 1);
     if (!g_dev[device]) {
         auto d = std::make_unique<DeviceShared>();
-        const auto &tt = triples();
-        d->triples = make_dev(tt.packed.size() * 4, "sort.triples");
-        HIP_CHECK(hipMemcpy(d->triples.get(), tt.packed.data(), tt.packed.size() * 4, hipMemcpyHostToDevice));
+
+
+
         HIP_CHECK(hipDeviceGetAttribute(&d->cus, hipDeviceAttributeMultiprocessorCount, device));
         int blocks = 0;
         HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, search_kernel, 256, 0));
@@ -282,6 +282,12 @@ DeviceShared &device_shared(int device) {
 }
 
 // ---------------------------------------------------------------- slots
+struct CaseInput {
+    const uint8_t *pi, *spi;
+    int n, e0, D, full;
+    float min_rate;
+};
+
 struct Slot {
     int device = -1;
     hipStream_t stream = nullptr;
@@ -308,6 +314,11 @@ struct Slot {
 
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     tdp_launch_stats stats{};
+
+    // configurations of the last batch, reused by TDP_FLAG_RESIDENT
+    int last_batch_count = -1;
+    std::vector<CaseInput> last_cases;
+    std::vector<int> last_index;
 };
 
 std::mutex g_slots_mutex;
@@ -409,11 +420,6 @@ void ensure_config_capacity(Slot &s, size_t count) {
     s.cfg_cap = cap;
 }
 
-struct CaseInput {
-    const uint8_t *pi, *spi;
-    int n, e0, D, full;
-    float min_rate;
-};
 
 void validate_case(const CaseInput &c) {
     if (c.n > kMaxN) throw std::invalid_argument("performGpuSortingSearch: n exceeds MAX_N");
@@ -439,7 +445,6 @@ void fill_config(ConfigDesc &d, const CaseInput &c) {
     d.max_depth = (uint32_t)c.D;
     d.e_start = (uint32_t)count_odd_cycles(c.spi, c.n);
     d.triple_count = (uint32_t)triple_count(c.n);
-    d.triple_offset = triples().offset[c.n];
     d.flags = c.full ? 1u : 0u;
     predicate_tables(c.n, c.e0, c.min_rate, c.D, c.full != 0, d.accept, d.reach);
     std::memcpy(d.spi0, c.spi, (size_t)c.n);
@@ -461,12 +466,13 @@ unsigned long long kernel_timeout_ticks() {
 // s.h_results; throws on cancel / overflow.
 void run_launch(Slot &s, const std::vector<CaseInput> &cases, uint32_t flags, bool single_search) {
     const int count = (int)cases.size();
+    const bool resident = (flags & TDP_FLAG_RESIDENT) != 0;
     HIP_CHECK(hipSetDevice(s.device));
     DeviceShared &dev = device_shared(s.device);
     ensure_config_capacity(s, (size_t)count);
     int max_n = 3;
     for (int c = 0; c < count; ++c) {
-        fill_config(as<ConfigDesc>(s.h_cfgs)[c], cases[c]);
+        if (!resident) fill_config(as<ConfigDesc>(s.h_cfgs)[c], cases[c]);
         max_n = std::max(max_n, cases[c].n);
     }
     const uint32_t nw = (uint32_t)node_words_for(max_n);
@@ -494,7 +500,7 @@ void run_launch(Slot &s, const std::vector<CaseInput> &cases, uint32_t flags, bo
         stack_cap = floor_pow2(stack_bytes / ((size_t)std::max(1, waves) * node_bytes));
     }
     if (stack_cap < 2) throw QueueOverflow("GPU sorting search queue overflow");
-    uint32_t pool_cap = donate ? std::min<uint32_t>(floor_pow2(pool_bytes / node_bytes), 1u << 20) : 0;
+    uint32_t pool_cap = donate ? floor_pow2(pool_bytes / node_bytes) : 0;
     if (donate && pool_cap < 64) {
         donate = false;
         pool_cap = 0;
@@ -520,7 +526,6 @@ void run_launch(Slot &s, const std::vector<CaseInput> &cases, uint32_t flags, bo
     P.results = as<ConfigResult>(s.results);
     P.num_configs = count;
     P.num_waves = waves;
-    P.triples = as<uint32_t>(dev.triples);
     P.stacks = as<uint32_t>(s.queue_mem);
     P.stack_cap = stack_cap;
     P.node_words = nw;
@@ -539,8 +544,9 @@ void run_launch(Slot &s, const std::vector<CaseInput> &cases, uint32_t flags, bo
     P.donate = donate ? 1 : 0;
     P.timeout_ticks = kernel_timeout_ticks();
 
-    HIP_CHECK(hipMemcpyAsync(s.cfgs.get(), s.h_cfgs.get(), (size_t)count * sizeof(ConfigDesc),
-                             hipMemcpyHostToDevice, s.stream));
+    if (!resident)
+        HIP_CHECK(hipMemcpyAsync(s.cfgs.get(), s.h_cfgs.get(), (size_t)count * sizeof(ConfigDesc),
+                                 hipMemcpyHostToDevice, s.stream));
     const size_t reset_n = std::max<size_t>({(size_t)count, (size_t)P.pool_cap, (size_t)kNumControl});
     hipLaunchKernelGGL(reset_kernel, dim3((unsigned)((reset_n + 255) / 256)), dim3(256), 0, s.stream, P);
     HIP_CHECK(hipGetLastError());
@@ -575,6 +581,14 @@ void run_launch(Slot &s, const std::vector<CaseInput> &cases, uint32_t flags, bo
     s.stats.donations = ctl[kCtlDonations].v;
     s.stats.steals = ctl[kCtlSteals].v;
     s.stats.node_bytes = node_bytes;
+    s.stats.pops = ctl[kCtlPops].v;
+    s.stats.pushes = ctl[kCtlPushes].v;
+    s.stats.probes = ctl[kCtlProbes].v;
+    s.stats.key_reads = ctl[kCtlKeyReads].v;
+    s.stats.key_writes = ctl[kCtlKeyWrites].v;
+    s.stats.key_bytes = (uint64_t)spi_words_for(max_n) * 4;
+    s.stats.overflow_info = ctl[kCtlOverflowInfo].v;
+    for (int k = 0; k < kNumPhases; ++k) s.stats.phase_ticks[k] = ctl[kCtlPhase0 + k].v;
 
     const ConfigResult *res = as<ConfigResult>(s.h_results);
     bool all_found = true;
@@ -835,16 +849,30 @@ int tdp_search_batch(int slot, int count, const uint8_t *pis, const uint8_t *spi
         std::vector<CaseInput> cases;
         std::vector<int> index;   // batch position of each launched case
         std::memset(out, 0, sizeof(tdp_case_result) * (size_t)count);
-        for (int c = 0; c < count; ++c) {
-            CaseInput ci{pis + 64 * (size_t)c, spis + 64 * (size_t)c, ns[c], e0s[c], max_moves[c],
-                         full_sorting[c], min_rates[c]};
-            validate_case(ci);
-            if (triple_count(ci.n) == 0) continue;
-            cases.push_back(ci);
-            index.push_back(c);
+        if (flags & TDP_FLAG_RESIDENT) {
+            if (count != s.last_batch_count || s.last_cases.empty())
+                throw std::invalid_argument("resident batch does not match the previous batch on this slot");
+            cases = s.last_cases;
+            index = s.last_index;
+        } else {
+            for (int c = 0; c < count; ++c) {
+                CaseInput ci{pis + 64 * (size_t)c, spis + 64 * (size_t)c, ns[c], e0s[c], max_moves[c],
+                             full_sorting[c], min_rates[c]};
+                validate_case(ci);
+                if (triple_count(ci.n) == 0) continue;
+                cases.push_back(ci);
+                index.push_back(c);
+            }
         }
         if (cases.empty()) return;
+        s.last_batch_count = -1;
         run_launch(s, cases, flags, false);
+        if (!(flags & TDP_FLAG_RESIDENT)) {
+            s.last_cases = cases;
+            for (auto &ci : s.last_cases) ci.pi = ci.spi = nullptr;   // only n is kept
+            s.last_index = index;
+        }
+        s.last_batch_count = count;
         const ConfigResult *res = as<ConfigResult>(s.h_results);
         for (size_t q = 0; q < cases.size(); ++q) {
             tdp_case_result &o = out[index[q]];
@@ -905,7 +933,7 @@ int tdp_eval_children(const uint8_t *pi, const uint8_t *spi, int n, int32_t *out
         HIP_CHECK(hipMemcpy(d_spi.get(), spi, (size_t)n, hipMemcpyHostToDevice));
         const int e = count_odd_cycles(spi, n);
         hipLaunchKernelGGL(eval_children_kernel, dim3(1), dim3(64), 0, nullptr, as<uint8_t>(d_pi), as<uint8_t>(d_spi),
-                           n, as<uint32_t>(ds.triples) + triples().offset[n], T, e, as<int32_t>(d_out));
+                           n, T, e, as<int32_t>(d_out));
         HIP_CHECK(hipGetLastError());
         HIP_CHECK(hipDeviceSynchronize());
         HIP_CHECK(hipMemcpy(out, d_out.get(), (size_t)T * 4, hipMemcpyDeviceToHost));

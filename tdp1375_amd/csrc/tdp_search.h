@@ -14,7 +14,6 @@ struct SearchParams {
     int num_configs;
     int num_waves;
 
-    const uint32_t *triples;      // packed triples of every n (offset per n)
 
     uint32_t *stacks;             // num_waves x stack_cap x node_words
     uint32_t stack_cap;           // power of two
@@ -41,7 +40,7 @@ struct SearchParams {
 
 __global__ void search_kernel(SearchParams P);
 __global__ void reset_kernel(SearchParams P);
-__global__ void eval_children_kernel(const uint8_t *pi, const uint8_t *spi, int n, const uint32_t *tri,
+__global__ void eval_children_kernel(const uint8_t *pi, const uint8_t *spi, int n,
                                      int T, int e, int32_t *out);
 
 }  // namespace tdp

@@ -37,17 +37,40 @@ namespace {
 
 constexpr int kWavesPerBlock = 4;
 constexpr int kStage = 512;
-constexpr int kMaxProbes = 4096;
+// Probe window of the dedup table: a child whose probe sequence finds neither
+// its key nor a free entry within this many slots is admitted without an
+// entry (the reference gives up after 8, sorting_search.cu:36,291-297); this
+// keeps a saturated table from stalling the search.
+constexpr int kMaxProbes = 64;
 
 enum DedupOutcome { kDedupPending = -1, kDedupDominated = 0, kDedupImproved = 1,
                     kDedupInserted = 2, kDedupUninserted = 3 };
 
+// Per-wave LDS slice.  The 64-bit counters live here rather than in
+// registers (lane 0 updates them with wave-uniform values).
 struct WaveLds {
     uint32_t node[48];
     uint32_t posinfo[64];
-    uint32_t scratch[64];
     uint32_t stage[kStage];
+    unsigned long long cnt[kNumCounters];    // counters of the current configuration
+    unsigned long long pend[kNumCounters];   // counters of the current expansion
+    unsigned long long traffic[5];           // pops, pushes, tag reads, key compares, key writes
+    unsigned long long phase[kNumPhases];    // 10 ns ticks per phase (TDP_PHASE_TIMING builds)
 };
+
+__device__ __forceinline__ void lds_add(unsigned long long *p, unsigned long long v) {
+    if ((threadIdx.x & 63) == 0) *p += v;
+}
+
+// Optional per-phase wall time (s_memrealtime, 100 MHz) for profiling builds
+// (-DTDP_PHASE_TIMING); it adds timing reads only and changes no work.
+#ifdef TDP_PHASE_TIMING
+#define PHASE_T0() const unsigned long long phase_t0_ = __builtin_amdgcn_s_memrealtime()
+#define PHASE_ADD(L, k) lds_add(&(L).phase[k], __builtin_amdgcn_s_memrealtime() - phase_t0_)
+#else
+#define PHASE_T0() (void)0
+#define PHASE_ADD(L, k) (void)0
+#endif
 
 __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -60,12 +83,29 @@ __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirs
 __device__ __forceinline__ unsigned long long ld_relaxed(const unsigned long long *p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-__device__ __forceinline__ unsigned long long ld_acquire(const unsigned long long *p) {
-    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+// Cross-wave data (dedup entries, pool nodes, flags) is only ever accessed
+// with relaxed device-scope atomics (sc1 loads and stores), which are
+// coherent per location across the XCDs' L2 caches.  Publication order comes
+// from waiting, in this wave, for every outstanding vector memory access to
+// complete before the flag access is issued (s_waitcnt vmcnt(0); gfx9 counts
+// stores in vmcnt too), so no L2 writeback or invalidate is ever needed.  A
+// workgroup-scope fence is not enough: on gfx950 it emits no vmcnt wait, and
+// two sc1 stores to different channels then become visible in either order.
+// The "memory" clobber keeps the compiler from moving accesses across it.
+__device__ __forceinline__ uint32_t ld32(const uint32_t *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-__device__ __forceinline__ void st_release(unsigned long long *p, unsigned long long v) {
-    __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+__device__ __forceinline__ void st32(uint32_t *p, uint32_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+__device__ __forceinline__ void st64(unsigned long long *p, unsigned long long v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void vm_complete() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
+// payload accesses complete before the following publishing store
+__device__ __forceinline__ void complete_before() { vm_complete(); }
+// the preceding load completes before any following access is issued
+__device__ __forceinline__ void complete_after() { vm_complete(); }
 __device__ __forceinline__ unsigned long long fetch_add(unsigned long long *p, unsigned long long v) {
     return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
@@ -108,6 +148,92 @@ __device__ __forceinline__ uint32_t child_spi_word(const uint32_t *pw, int q, co
     return w;
 }
 
+// Triple enumeration without memory traffic.  In descending lexicographic
+// order the triples with first index i = n-1-m occupy offsets
+// [C(m,3), C(m+1,3)), and inside that block the pairs (j, k) follow the colex
+// order of (u, v) = (n-1-k, n-1-j), the same sequence for every i and n.  One
+// table of the C(63,2) colex pairs (u | v << 8) per workgroup in LDS therefore
+// yields any triple from its offset o; the lexicographic index is
+// t = C(n,3) - 1 - o.
+constexpr int kColexPairs = 1953;   // C(63, 2)
+
+__device__ __forceinline__ int choose3(int m) { return m * (m - 1) * (m - 2) / 6; }
+__device__ __forceinline__ int choose2(int v) { return v * (v - 1) / 2; }
+
+__device__ __forceinline__ int lex_index(int n, int T, int i, int j, int k) {
+    return T - 1 - choose3(n - 1 - i) - choose2(n - 1 - j) - (n - 1 - k);
+}
+
+// Triple at descending offset o; m is the caller's block cursor, advanced to
+// the block holding o (offsets only grow along a lane's scan).
+__device__ __forceinline__ void triple_at(const uint16_t *colex, int n, int &m, int o, bool valid, int &i,
+                                          int &j, int &k) {
+    while (m + 1 < n && choose3(m + 1) <= o) ++m;
+    const uint32_t pair = colex[valid ? o - choose3(m) : 0];
+    i = n - 1 - m;
+    j = n - 1 - (int)(pair >> 8);
+    k = n - 1 - (int)(pair & 0xFF);
+}
+
+// Parity delta of one child from the posinfo of its three symbols: three
+// different cycles merge into one (delta from the lengths only); the cases
+// where a cycle is hit twice take the full rule.
+__device__ __forceinline__ bool hits_a_cycle_twice(uint32_t ia, uint32_t ib, uint32_t ic) {
+    return ((ia ^ ib) & 63u) == 0 || ((ib ^ ic) & 63u) == 0 || ((ia ^ ic) & 63u) == 0;
+}
+
+__device__ __forceinline__ int child_delta(uint32_t ia, uint32_t ib, uint32_t ic) {
+    // three cycles merge into one: -2 when at least two of them are odd, else 0
+    const uint32_t odd = ((ia >> 16) & 1u) + ((ib >> 16) & 1u) + ((ic >> 16) & 1u);
+    int delta = odd >= 2u ? -2 : 0;
+    if (hits_a_cycle_twice(ia, ib, ic)) delta = parity_delta(ia, ib, ic);
+    return delta;
+}
+
+// Enumerates the children in descending lexicographic order, one chunk of
+// 64 consecutive offsets per call of visit(valid, i, j, k, obase) (lane l
+// holds offset obase + l); visit returns true to stop.  Blocks m <= 11
+// (first index i = n-1-m) hold fewer than 64 pairs each, so the first
+// C(12,3) = 220 offsets run with a per-lane block cursor; larger blocks run
+// one wave-uniform block at a time with lanes over its colex pairs.
+template <typename Visit>
+__device__ __forceinline__ void for_each_child_chunk(const uint16_t *colex, int n, int T, int lane, Visit visit) {
+    const int prefix = T < 220 ? T : 220;
+    bool done = false;
+    int m = 2;
+    for (int obase = 0; obase < prefix && !done; obase += kWave) {
+        const int o = obase + lane;
+        const bool valid = o < prefix;
+        int i, j, k;
+        triple_at(colex, n, m, o, valid, i, j, k);
+        done = visit(valid, i, j, k, obase);
+    }
+    for (int mb = 12; mb < n && !done; ++mb) {
+        const int pairs = choose2(mb), o0 = choose3(mb), i = n - 1 - mb;
+        for (int qb = 0; qb < pairs && !done; qb += kWave) {
+            const int q = qb + lane;
+            const bool valid = q < pairs;
+            const uint32_t pair = colex[valid ? q : 0];
+            done = visit(valid, i, n - 1 - (int)(pair >> 8), n - 1 - (int)(pair & 0xFF), o0 + qb);
+        }
+    }
+}
+
+__device__ void fill_colex(uint16_t *colex) {
+    // q -> (u, v) with v the largest such that C(v,2) <= q
+    for (int q = threadIdx.x; q < kColexPairs; q += blockDim.x) {
+        int v = 1;
+        while (choose2(v + 1) <= q) ++v;
+        colex[q] = (uint16_t)((q - choose2(v)) | (v << 8));
+    }
+    __syncthreads();
+}
+
+// Stage entries: i | j << 6 | k << 12 | even << 18.
+__device__ __forceinline__ uint32_t stage_entry(int i, int j, int k, int even) {
+    return (uint32_t)i | ((uint32_t)j << 6) | ((uint32_t)k << 12) | ((uint32_t)even << 18);
+}
+
 // transposition.cuh:5-11
 __device__ __forceinline__ int transposed_src_index(int p, int i, int j, int k) {
     const int lenA = j - i, lenB = k - j;
@@ -141,9 +267,10 @@ __device__ void build_posinfo(WaveLds &L, int n, int sw, int lane) {
         f += fq;
         q = qq;
     }
-    if (nxt == root) L.scratch[root] = (uint32_t)(f + 1);
-    wave_sync();
-    const int len = (int)L.scratch[root];
+    // f is the number of steps to the root, so the root's successor has
+    // f = len - 1 (a fixed point is its own successor with f = 0)
+    const int f_next = __shfl(f, nxt);
+    const int len = __shfl(f_next + 1, root);
     const int pos = (f == 0) ? 0 : len - f;
     const uint32_t info = cycle_info(root, pos, len);
     const int x = (s < n) ? (int)pi[s] : s;
@@ -151,15 +278,11 @@ __device__ void build_posinfo(WaveLds &L, int n, int sw, int lane) {
     wave_sync();
 }
 
-struct Counters {
-    unsigned long long v[kNumCounters];
-};
-
 struct WaveState {
     int wave, lane;
+    const uint16_t *colex;         // workgroup's colex pair table (LDS)
     uint32_t lo, hi;               // private stack window [lo, hi), wraps mod 2^32
     int cur_cfg;
-    Counters cnt;
     bool configs_left;
     bool stop;
     unsigned long long deadline;   // s_memrealtime ticks (100 MHz); 0 = none
@@ -174,17 +297,14 @@ __device__ void set_watchdog(const SearchParams &P, WaveState &W) {
     W.stop = true;
 }
 
-__device__ void flush_counters(const SearchParams &P, WaveState &W) {
-    if (W.cur_cfg >= 0) {
-        if (W.lane < kNumCounters) {
-            unsigned long long v = 0;
-#pragma unroll
-            for (int c = 0; c < kNumCounters; ++c) v = (W.lane == c) ? W.cnt.v[c] : v;
-            if (v) fetch_add(&P.results[W.cur_cfg].counters[W.lane], v);
-        }
+__device__ void flush_counters(const SearchParams &P, WaveState &W, WaveLds &L) {
+    wave_sync();
+    if (W.cur_cfg >= 0 && W.lane < kNumCounters) {
+        const unsigned long long v = L.cnt[W.lane];
+        if (v) fetch_add(&P.results[W.cur_cfg].counters[W.lane], v);
     }
-#pragma unroll
-    for (int c = 0; c < kNumCounters; ++c) W.cnt.v[c] = 0;
+    if (W.lane < kNumCounters) L.cnt[W.lane] = 0;
+    wave_sync();
 }
 
 __device__ __forceinline__ uint32_t *stack_slot(const SearchParams &P, const WaveState &W, uint32_t idx) {
@@ -202,11 +322,19 @@ __device__ bool donate_one(const SearchParams &P, WaveState &W) {
     unsigned long long ticket = 0;
     int ok = 0;
     if (W.lane == 0) {
-        for (int attempt = 0; attempt < 64; ++attempt) {
-            const unsigned long long t = ctl_load(P, kCtlPoolTail);
+        // a lost CAS only means another wave moved the tail: retry until the
+        // pool is really full (or the watchdog fires)
+        for (unsigned attempt = 0;; ++attempt) {
+            // head first, and complete before the tail is read: loads to
+            // different lines complete in any order, and a head newer than
+            // the tail would make an empty pool look full
             const unsigned long long h = ctl_load(P, kCtlPoolHead);
-            if (t - h >= P.pool_cap) break;
+            complete_after();
+            const unsigned long long t = ctl_load(P, kCtlPoolTail);
+            if (t >= h && t - h >= P.pool_cap) break;
             if (cas64(&P.ctrl[kCtlPoolTail].v, t, t + 1)) { ticket = t; ok = 1; break; }
+            if ((attempt & 63) == 63 && past_deadline(W)) break;
+            __builtin_amdgcn_s_sleep(1);
         }
     }
     ok = __shfl(ok, 0);
@@ -216,18 +344,18 @@ __device__ bool donate_one(const SearchParams &P, WaveState &W) {
     const size_t slot = (size_t)(ticket & (P.pool_cap - 1));
     int late = 0;
     if (W.lane == 0) {
-        while (ld_acquire(&P.pool_seq[slot]) != ticket) {
+        while (ld_relaxed(&P.pool_seq[slot]) != ticket) {
             if (past_deadline(W)) { late = 1; break; }
             __builtin_amdgcn_s_sleep(2);
         }
     }
     if (__shfl(late, 0)) { set_watchdog(P, W); return false; }
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    complete_after();
     const uint32_t *src = stack_slot(P, W, W.lo);
     uint32_t *dst = P.pool_nodes + slot * P.node_words;
-    if (W.lane < (int)P.node_words) dst[W.lane] = src[W.lane];
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    if (W.lane == 0) st_release(&P.pool_seq[slot], ticket + 1);
+    if (W.lane < (int)P.node_words) st32(dst + W.lane, src[W.lane]);
+    complete_before();
+    if (W.lane == 0) st64(&P.pool_seq[slot], ticket + 1);
     W.lo++;
     return true;
 }
@@ -239,6 +367,7 @@ __device__ bool take_one(const SearchParams &P, WaveState &W, WaveLds &L) {
     if (W.lane == 0) {
         for (int attempt = 0; attempt < 8; ++attempt) {
             const unsigned long long h = ctl_load(P, kCtlPoolHead);
+            complete_after();
             const unsigned long long t = ctl_load(P, kCtlPoolTail);
             if (h >= t) break;
             if (cas64(&P.ctrl[kCtlPoolHead].v, h, h + 1)) { ticket = h; ok = 1; break; }
@@ -251,18 +380,18 @@ __device__ bool take_one(const SearchParams &P, WaveState &W, WaveLds &L) {
     const size_t slot = (size_t)(ticket & (P.pool_cap - 1));
     int late = 0;
     if (W.lane == 0) {
-        while (ld_acquire(&P.pool_seq[slot]) != ticket + 1) {
+        while (ld_relaxed(&P.pool_seq[slot]) != ticket + 1) {
             if (past_deadline(W)) { late = 1; break; }
             __builtin_amdgcn_s_sleep(2);
         }
     }
     if (__shfl(late, 0)) { set_watchdog(P, W); return false; }
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    complete_after();
     const uint32_t *src = P.pool_nodes + slot * P.node_words;
-    if (W.lane < (int)P.node_words) L.node[W.lane] = src[W.lane];
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    if (W.lane < (int)P.node_words) L.node[W.lane] = ld32(src + W.lane);
+    complete_before();
     if (W.lane == 0) {
-        st_release(&P.pool_seq[slot], ticket + P.pool_cap);
+        st64(&P.pool_seq[slot], ticket + P.pool_cap);
         fetch_add(&P.ctrl[kCtlSteals].v, 1ULL);
     }
     wave_sync();
@@ -272,11 +401,21 @@ __device__ bool take_one(const SearchParams &P, WaveState &W, WaveLds &L) {
 // Ensures room for one push on the private stack.
 __device__ bool make_room(const SearchParams &P, WaveState &W) {
     if (W.hi - W.lo < P.stack_cap) return true;
+    unsigned long long reason = 1;   // 1: no sharing, 2: pool full, 3: stack still full
     if (P.donate) {
-        for (int r = 0; r < 32 && W.hi - W.lo > 1; ++r)
+        int r = 0;
+        for (; r < 32 && W.hi - W.lo > 1; ++r)
             if (!donate_one(P, W)) break;
         if (W.stop) return false;
         if (W.hi - W.lo < P.stack_cap) return true;
+        reason = r < 32 ? 2 : 3;
+    }
+    if (W.lane == 0) {
+        const unsigned long long h = ctl_load(P, kCtlPoolHead);
+        complete_after();
+        const unsigned long long t = ctl_load(P, kCtlPoolTail);
+        st64(&P.ctrl[kCtlOverflowInfo].v, (reason << 56) | ((t - h) & 0xFFFFFFFFULL) |
+                                              ((unsigned long long)(W.hi - W.lo) << 32));
     }
     set_overflow(P, W);
     return false;
@@ -319,12 +458,23 @@ __device__ void push_child(const SearchParams &P, WaveState &W, WaveLds &L, int 
         dst[lane] = w;
     }
     W.hi++;
+    lds_add(&L.traffic[1], 1);
 }
 
 // Lane-parallel exact dedup (sorting_search.cu:261-298 semantics with an
 // exact table): returns a DedupOutcome for active lanes.
+struct LaneTraffic {
+    int probes, key_reads, key_writes;
+};
+
+__device__ __forceinline__ int wave_sum(int v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    return v;
+}
+
 __device__ int dedup_admit(const SearchParams &P, bool active, uint32_t cfg, const uint32_t *pw,
-                           int nw, const ChildMove &m, int depth) {
+                           int nw, const ChildMove &m, int depth, LaneTraffic &tr) {
     if (P.dd_cap == 0) return active ? kDedupUninserted : kDedupPending;
     uint64_t h = 0x9E3779B97F4A7C15ULL * (uint64_t)(cfg + 1);
     for (int q = 0; q < nw; ++q) {
@@ -342,21 +492,26 @@ __device__ int dedup_admit(const SearchParams &P, bool active, uint32_t cfg, con
     while (__ballot(res == kDedupPending) != 0) {
         if (res == kDedupPending) {
             const unsigned long long tg = ld_relaxed(&P.dd_tag[slot]);
+            tr.probes++;
             bool advance = false;
             if ((tg >> 48) != epoch) {
                 if (cas64(&P.dd_tag[slot], tg, mytag)) {
-                    P.dd_cfg[slot] = cfg;
+                    st32(&P.dd_cfg[slot], cfg);
                     uint32_t *kw = P.dd_key + slot * 16;
-                    for (int q = 0; q < nw; ++q) kw[q] = child_spi_word(pw, q, m);
-                    __hip_atomic_store(&P.dd_best[slot], bword, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                    for (int q = 0; q < nw; ++q) st32(kw + q, child_spi_word(pw, q, m));
+                    complete_before();
+                    st32(&P.dd_best[slot], bword);
                     res = kDedupInserted;
+                    tr.key_writes++;
                 }
             } else if (tg == mytag) {
-                const uint32_t bw = __hip_atomic_load(&P.dd_best[slot], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+                const uint32_t bw = ld32(&P.dd_best[slot]);
+                complete_after();
                 if ((bw >> 16) == (uint32_t)epoch) {
-                    bool eq = P.dd_cfg[slot] == cfg;
+                    tr.key_reads++;
+                    bool eq = ld32(&P.dd_cfg[slot]) == cfg;
                     const uint32_t *kw = P.dd_key + slot * 16;
-                    for (int q = 0; q < nw && eq; ++q) eq = kw[q] == child_spi_word(pw, q, m);
+                    for (int q = 0; q < nw && eq; ++q) eq = ld32(kw + q) == child_spi_word(pw, q, m);
                     if (eq) {
                         const uint32_t old = __hip_atomic_fetch_min(&P.dd_best[slot], bword, __ATOMIC_RELAXED,
                                                                     __HIP_MEMORY_SCOPE_AGENT);
@@ -380,7 +535,7 @@ __device__ int dedup_admit(const SearchParams &P, bool active, uint32_t cfg, con
 // Deduplicates and pushes the staged children (descending triple order).
 // root_stage: children of the start node (not counted as admitted).
 __device__ void flush_stage(const SearchParams &P, WaveState &W, WaveLds &L, int count, int n, int sw,
-                            const uint32_t *tri, Counters &pending, bool root_stage) {
+                            int T, bool root_stage) {
     const uint32_t cfg = L.node[0];
     const int d = (int)(L.node[1] & 0xFF);
     const uint8_t *spi = reinterpret_cast<const uint8_t *>(L.node + kNodeHeaderWords);
@@ -388,35 +543,44 @@ __device__ void flush_stage(const SearchParams &P, WaveState &W, WaveLds &L, int
     for (int base = 0; base < count && !W.stop; base += kWave) {
         const int idx = base + W.lane;
         const bool act = idx < count;
-        const uint32_t st = act ? L.stage[idx] : 0;
-        const int t = (int)(st & 0xFFFF);
-        const int ce = (int)(st >> 16);
-        const uint32_t tp = act ? tri[t] : 0;
-        const int i = tp & 0xFF, j = (tp >> 8) & 0xFF, k = (tp >> 16) & 0xFF;
+        const uint32_t st = act ? L.stage[idx] : stage_entry(0, 1, 2, 0);
+        const int i = st & 63, j = (st >> 6) & 63, k = (st >> 12) & 63;
+        const int ce = (int)(st >> 18);
+        const int t = lex_index(n, T, i, j, k);
         ChildMove m;
         m.a = pi[i]; m.b = pi[j]; m.c = pi[k];
         m.na = spi[m.c]; m.nb = spi[m.a]; m.nc = spi[m.b];
-        const int r = dedup_admit(P, act, cfg, L.node + kNodeHeaderWords, sw, m, d + 1);
+        LaneTraffic tr{0, 0, 0};
+        PHASE_T0();
+        const int r = dedup_admit(P, act, cfg, L.node + kNodeHeaderWords, sw, m, d + 1, tr);
+        PHASE_ADD(L, kPhaseDedup);
+        lds_add(&L.traffic[2], (unsigned long long)wave_sum(tr.probes));
+        lds_add(&L.traffic[3], (unsigned long long)wave_sum(tr.key_reads));
+        lds_add(&L.traffic[4], (unsigned long long)wave_sum(tr.key_writes));
         const bool admit = act && r != kDedupDominated;
         if (!root_stage) {
             const unsigned long long dom = __popcll(__ballot(act && r == kDedupDominated));
             const unsigned long long imp = __popcll(__ballot(act && r == kDedupImproved));
             const unsigned long long ins = __popcll(__ballot(act && r == kDedupInserted));
             const unsigned long long uni = __popcll(__ballot(act && r == kDedupUninserted));
-            pending.v[kCntDominated] += dom;
-            pending.v[kCntImproved] += imp;
-            pending.v[kCntInserted] += ins;
-            pending.v[kCntUninserted] += uni;
-            pending.v[kCntAdmitted] += imp + ins + uni;
+            lds_add(&L.pend[kCntDominated], dom);
+            lds_add(&L.pend[kCntImproved], imp);
+            lds_add(&L.pend[kCntInserted], ins);
+            lds_add(&L.pend[kCntUninserted], uni);
+            lds_add(&L.pend[kCntAdmitted], imp + ins + uni);
         }
         unsigned long long pm = __ballot(admit);
-        while (pm) {
-            const int src = __ffsll((long long)pm) - 1;
-            pm &= pm - 1;
-            const int ct = __shfl(t, src), ci = __shfl(i, src), cj = __shfl(j, src), ck = __shfl(k, src);
-            const int cce = __shfl(ce, src);
-            push_child(P, W, L, n, sw, uniform(ct), uniform(ci), uniform(cj), uniform(ck), uniform(cce));
-            if (W.stop) break;
+        {
+            PHASE_T0();
+            while (pm) {
+                const int src = __ffsll((long long)pm) - 1;
+                pm &= pm - 1;
+                const int ct = __shfl(t, src), ci = __shfl(i, src), cj = __shfl(j, src), ck = __shfl(k, src);
+                const int cce = __shfl(ce, src);
+                push_child(P, W, L, n, sw, uniform(ct), uniform(ci), uniform(cj), uniform(ck), uniform(cce));
+                if (W.stop) break;
+            }
+            PHASE_ADD(L, kPhasePush);
         }
     }
 }
@@ -456,60 +620,96 @@ __device__ int expand(const SearchParams &P, WaveState &W, WaveLds &L, const Con
     const int D = (int)C.max_depth;
     const int d = (int)(L.node[1] & 0xFF);
     const int e = (int)((L.node[1] >> 8) & 0xFF);
-    const uint32_t *tri = P.triples + C.triple_offset;
-    build_posinfo(L, n, sw, W.lane);
+    {
+        PHASE_T0();
+        build_posinfo(L, n, sw, W.lane);
+        PHASE_ADD(L, kPhaseAnalyse);
+    }
+    // Both predicates are monotone in the odd-cycle count and a move changes it
+    // by -2, 0 or +2, so each expansion reduces to two 3-bit masks over
+    // delta in {-2, 0, +2}: which deltas make a goal child, which make an
+    // admissible one.
     const uint64_t acc = C.accept[d + 1];
     const uint64_t rch = (d + 1 < D) ? C.reach[d + 1] : 0ULL;
-    Counters pending;
+    uint32_t gmask = 0, amask = 0;
 #pragma unroll
-    for (int c = 0; c < kNumCounters; ++c) pending.v[c] = 0;
+    for (int q = 0; q < 3; ++q) {
+        const int ev = e + 2 * q - 2;
+        if (ev >= 0 && ev < 64) {
+            gmask |= (uint32_t)((acc >> ev) & 1ULL) << q;
+            amask |= (uint32_t)((rch >> ev) & 1ULL) << q;
+        }
+    }
+    if (root_mode) {
+        amask |= gmask;   // live roots: goals or can_still_reach at depth 1
+        gmask = 0;
+    }
+    if (gmask == 0 && amask == 0) return -1;   // no child can be a goal or admitted
+    if (W.lane < kNumCounters) L.pend[W.lane] = 0;
+    wave_sync();
     int scount = 0;
     int goal_t = -1;
-    for (int base = T - 1; base >= 0; base -= kWave) {
-        const int t = base - W.lane;
-        const bool valid = t >= 0;
-        const uint32_t tp = valid ? tri[t] : 0;
-        const int ce = e + parity_delta(L.posinfo[tp & 0xFF], L.posinfo[(tp >> 8) & 0xFF],
-                                        L.posinfo[(tp >> 16) & 0xFF]);
-        const bool goal = valid && ((acc >> ce) & 1ULL);
-        const bool adm = valid && ((rch >> ce) & 1ULL);
+    PHASE_T0();
+    // When only delta = +2 can qualify, a child must hit one cycle twice (three
+    // different cycles merge, delta <= 0): chunks without such a child are
+    // skipped wave-wide.
+    const bool plus2_only = (gmask | amask) == 4u;
+    // Visits the chunk whose lane l holds the child at descending offset
+    // obase + l (lexicographic index T-1-obase-l); true ends the scan.
+    auto visit = [&](bool valid, int i, int j, int k, int obase) -> bool {
+        const uint32_t ia = L.posinfo[i], ib = L.posinfo[j], ic = L.posinfo[k];
+        int delta;
+        if (plus2_only) {
+            const bool shared = hits_a_cycle_twice(ia, ib, ic);
+            if (__ballot(valid && shared) == 0) return false;
+            delta = shared ? parity_delta(ia, ib, ic) : -2;
+        } else {
+            delta = child_delta(ia, ib, ic);
+        }
+        const uint32_t di = (uint32_t)(delta + 2) >> 1;
+        const bool goal = valid && ((gmask >> di) & 1u);
+        const bool adm = valid && ((amask >> di) & 1u);
         if (!root_mode) {
             const unsigned long long gm = __ballot(goal);
             if (gm) {
-                goal_t = base - (__ffsll((long long)gm) - 1);
-                break;
+                goal_t = T - 1 - (obase + __ffsll((long long)gm) - 1);
+                return true;
             }
         }
-        const unsigned long long am = __ballot(root_mode ? (goal || adm) : adm);
+        const unsigned long long am = __ballot(adm);
         if (am) {
             const unsigned long long below = (W.lane == 0) ? 0ULL : (am & (~0ULL >> (64 - W.lane)));
-            if ((root_mode ? (goal || adm) : adm))
-                L.stage[scount + __popcll(below)] = (uint32_t)t | ((uint32_t)ce << 16);
+            if (adm) L.stage[scount + __popcll(below)] = stage_entry(i, j, k, e + delta);
             scount += __popcll(am);
-            if (root_mode) W.cnt.v[kCntRootsLive] += __popcll(am);
+            if (root_mode) lds_add(&L.cnt[kCntRootsLive], __popcll(am));
             if (scount > kStage - kWave) {
                 wave_sync();
-                flush_stage(P, W, L, scount, n, sw, tri, pending, root_mode);
+                flush_stage(P, W, L, scount, n, sw, T, root_mode);
                 scount = 0;
-                if (W.stop) return -1;
+                if (W.stop) return true;
             }
         }
-    }
+        return false;
+    };
+    for_each_child_chunk(W.colex, n, T, W.lane, visit);
+    if (W.stop) return -1;
+    PHASE_ADD(L, kPhaseScan);
     if (goal_t >= 0) return goal_t;
     if (scount) {
         wave_sync();
-        flush_stage(P, W, L, scount, n, sw, tri, pending, root_mode);
+        flush_stage(P, W, L, scount, n, sw, T, root_mode);
     }
     if (!root_mode) {
-#pragma unroll
-        for (int c = 0; c < kNumCounters; ++c) W.cnt.v[c] += pending.v[c];
+        wave_sync();
+        if (W.lane < kNumCounters) L.cnt[W.lane] += L.pend[W.lane];
+        wave_sync();
     }
     return -1;
 }
 
-__device__ void switch_config(const SearchParams &P, WaveState &W, int cfg) {
+__device__ void switch_config(const SearchParams &P, WaveState &W, WaveLds &L, int cfg) {
     if (cfg != W.cur_cfg) {
-        flush_counters(P, W);
+        flush_counters(P, W, L);
         W.cur_cfg = cfg;
     }
 }
@@ -533,36 +733,46 @@ __device__ bool should_stop(const SearchParams &P) {
 
 __global__ __launch_bounds__(256) void search_kernel(SearchParams P) {
     __shared__ WaveLds lds[kWavesPerBlock];
+    __shared__ uint16_t colex[kColexPairs];
+    fill_colex(colex);
     const int wib = threadIdx.x >> 6;
     WaveLds &L = lds[wib];
     WaveState W;
     W.lane = threadIdx.x & 63;
     W.wave = blockIdx.x * kWavesPerBlock + wib;
     if (W.wave >= P.num_waves) return;
+    W.colex = colex;
     W.lo = W.hi = 0;
     W.cur_cfg = -1;
     W.configs_left = true;
     W.stop = false;
     W.deadline = P.timeout_ticks ? __builtin_amdgcn_s_memrealtime() + P.timeout_ticks : 0ULL;
-#pragma unroll
-    for (int c = 0; c < kNumCounters; ++c) W.cnt.v[c] = 0;
+    if (W.lane < kNumCounters) L.cnt[W.lane] = 0;
+    if (W.lane < 5) L.traffic[W.lane] = 0;
+    if (W.lane < kNumPhases) L.phase[W.lane] = 0;
+    wave_sync();
     if (W.lane == 0) fetch_add(&P.ctrl[kCtlRegistered].v, 1ULL);
     bool idle = false;
     unsigned iter = 0;
+    unsigned polls = 0, idle_polls = 0;
+    int backoff = 1;
 
     while (true) {
-        if (should_stop(P) || W.stop) break;
+        if (W.stop || ((++polls & 7) == 0 && should_stop(P))) break;
         if (past_deadline(W)) {
             set_watchdog(P, W);
             break;
         }
         bool have = false;
         if (W.hi > W.lo) {
+            PHASE_T0();
             W.hi--;
+            lds_add(&L.traffic[0], 1);
             const uint32_t *src = stack_slot(P, W, W.hi);
             if (W.lane < (int)P.node_words) L.node[W.lane] = src[W.lane];
             wave_sync();
             have = true;
+            PHASE_ADD(L, kPhasePop);
         } else if (W.configs_left) {
             int c = 0;
             if (W.lane == 0) c = (int)fetch_add(&P.ctrl[kCtlNextConfig].v, 1ULL);
@@ -572,7 +782,7 @@ __global__ __launch_bounds__(256) void search_kernel(SearchParams P) {
                 continue;
             }
             // start node of configuration c, expanded in root mode
-            switch_config(P, W, c);
+            switch_config(P, W, L, c);
             const ConfigDesc &C = P.cfgs[c];
             const int sw = (int)P.spi_words;
             if (W.lane < (int)P.node_words) {
@@ -586,37 +796,63 @@ __global__ __launch_bounds__(256) void search_kernel(SearchParams P) {
                 L.node[W.lane] = w;
             }
             wave_sync();
-            if (C.triple_count > 0) expand(P, W, L, C, true);
+            if (C.triple_count > 0) {
+                PHASE_T0();
+                expand(P, W, L, C, true);
+                PHASE_ADD(L, kPhaseRoot);
+            }
             continue;
         } else if (P.donate) {
+            PHASE_T0();
             if (!idle) {
                 if (W.lane == 0) fetch_add(&P.ctrl[kCtlIdle].v, 1ULL);
                 idle = true;
             }
-            const unsigned long long takes0 = ctl_load(P, kCtlTakes);
-            const unsigned long long head = ctl_load(P, kCtlPoolHead);
-            const unsigned long long tail = ctl_load(P, kCtlPoolTail);
-            if (head < tail) {
+            // Thousands of idle waves poll the same control lines, so a poll
+            // is two loads by lane 0 with up to ~60 us of sleep between polls,
+            // and the termination test runs on every 4th empty poll only.
+            // Termination: the pool is empty at the first read (head == tail),
+            // every registered wave is idle at the second, and neither head nor
+            // tail moved by the third.  Work only leaves a wave through the
+            // pool (tail moves) and only enters an idle wave by a take (head
+            // moves, after the taker left the idle count), so no wave can hold
+            // a node once all three reads agree.
+            int work = 0, done = 0;
+            if (W.lane == 0) {
+                const unsigned long long h0 = ctl_load(P, kCtlPoolHead);
+                complete_after();
+                const unsigned long long t0 = ctl_load(P, kCtlPoolTail);
+                work = h0 < t0 ? 1 : 0;
+                if (!work && (++idle_polls & 3) == 0) {
+                    complete_after();
+                    const unsigned long long idl = ctl_load(P, kCtlIdle);
+                    complete_after();
+                    const unsigned long long reg = ctl_load(P, kCtlRegistered);
+                    complete_after();
+                    const unsigned long long h2 = ctl_load(P, kCtlPoolHead);
+                    complete_after();
+                    const unsigned long long t2 = ctl_load(P, kCtlPoolTail);
+                    done = (idl == reg && h2 == h0 && t2 == t0) ? 1 : 0;
+                }
+            }
+            idle_polls = __shfl(idle_polls, 0);
+            if (__shfl(work, 0)) {
                 if (W.lane == 0) fetch_add(&P.ctrl[kCtlIdle].v, ~0ULL);
                 idle = false;
+                backoff = 1;
                 if (take_one(P, W, L)) {
-                    if (W.lane == 0) fetch_add(&P.ctrl[kCtlTakes].v, 1ULL);
                     have = true;
+                    PHASE_ADD(L, kPhaseShare);
                 } else {
+                    PHASE_ADD(L, kPhaseShare);
                     continue;
                 }
             } else {
-                int done = 0;
-                if (W.lane == 0) {
-                    const unsigned long long idl = ctl_load(P, kCtlIdle);
-                    const unsigned long long reg = ctl_load(P, kCtlRegistered);
-                    const unsigned long long h2 = ctl_load(P, kCtlPoolHead);
-                    const unsigned long long t2 = ctl_load(P, kCtlPoolTail);
-                    const unsigned long long takes1 = ctl_load(P, kCtlTakes);
-                    done = (idl == reg && h2 == t2 && takes1 == takes0) ? 1 : 0;
-                }
                 if (__shfl(done, 0)) break;
-                __builtin_amdgcn_s_sleep(8);
+                if ((idle_polls & 3) == 0 && should_stop(P)) break;
+                for (int z = 0; z < backoff; ++z) __builtin_amdgcn_s_sleep(127);
+                if (backoff < 16) backoff *= 2;
+                PHASE_ADD(L, kPhaseShare);
                 continue;
             }
         } else {
@@ -626,37 +862,56 @@ __global__ __launch_bounds__(256) void search_kernel(SearchParams P) {
 
         // ---- process the node in L.node
         const int cfg = uniform((int)L.node[0]);
-        switch_config(P, W, cfg);
-        if (config_found(P, cfg)) continue;
         const ConfigDesc &C = P.cfgs[cfg];
         const int d = uniform((int)(L.node[1] & 0xFF));
         const int e = uniform((int)((L.node[1] >> 8) & 0xFF));
+        {
+            PHASE_T0();
+            switch_config(P, W, L, cfg);
+            const bool skip = config_found(P, cfg);
+            PHASE_ADD(L, kPhaseCheck);
+            if (skip) continue;
+        }
         if ((C.accept[d] >> e) & 1ULL) {
             report_found(P, W, L, cfg, d, -1);
             continue;
         }
         if (d >= (int)C.max_depth || !((C.reach[d] >> e) & 1ULL)) continue;
-        W.cnt.v[kCntExpansions]++;
-        W.cnt.v[kCntGenerated] += C.triple_count;
+        lds_add(&L.cnt[kCntExpansions], 1);
+        lds_add(&L.cnt[kCntGenerated], C.triple_count);
         const int goal = expand(P, W, L, C, false);
         if (goal >= 0) report_found(P, W, L, cfg, d + 1, goal);
 
-        // donation: give the oldest half of the stack to idle waves
-        if (P.donate && ((++iter & 3) == 0) && W.hi - W.lo >= 2) {
+        // donation: when other waves are idle, hand a few of the oldest nodes
+        // (the largest subtrees) to the shared pool; checked every 16
+        // expansions so the owner's depth-first order is mostly kept
+        if (P.donate && ((++iter & 15) == 0) && W.hi - W.lo >= 3) {
+            PHASE_T0();
             unsigned long long idl = 0;
             if (W.lane == 0) idl = ctl_load(P, kCtlIdle);
             idl = (unsigned long long)__shfl((int)idl, 0);
             if (idl > 0) {
                 uint32_t give = (W.hi - W.lo) / 2;
                 if (give > idl) give = (uint32_t)idl;
-                if (give > 16) give = 16;
+                if (give > 4) give = 4;
                 for (uint32_t g = 0; g < give; ++g)
                     if (!donate_one(P, W)) break;
                 if (W.lane == 0) fetch_add(&P.ctrl[kCtlDonations].v, (unsigned long long)give);
             }
+            PHASE_ADD(L, kPhaseShare);
         }
     }
-    flush_counters(P, W);
+    flush_counters(P, W, L);
+#ifdef TDP_PHASE_TIMING
+    if (W.lane < kNumPhases) fetch_add(&P.ctrl[kCtlPhase0 + W.lane].v, L.phase[W.lane]);
+#endif
+    if (W.lane == 0) {
+        fetch_add(&P.ctrl[kCtlPops].v, L.traffic[0]);
+        fetch_add(&P.ctrl[kCtlPushes].v, L.traffic[1]);
+        fetch_add(&P.ctrl[kCtlProbes].v, L.traffic[2]);
+        fetch_add(&P.ctrl[kCtlKeyReads].v, L.traffic[3]);
+        fetch_add(&P.ctrl[kCtlKeyWrites].v, L.traffic[4]);
+    }
 }
 
 // Reset of the launch state (replaces the per-search memsets of
@@ -676,10 +931,11 @@ __global__ void reset_kernel(SearchParams P) {
 
 // Debug/validation kernel: odd-cycle count of every child of one node,
 // computed with the same LDS cycle analysis and parity delta as the search.
-__global__ __launch_bounds__(64) void eval_children_kernel(const uint8_t *pi, const uint8_t *spi, int n,
-                                                           const uint32_t *tri, int T, int e,
-                                                           int32_t *out) {
+__global__ __launch_bounds__(64) void eval_children_kernel(const uint8_t *pi, const uint8_t *spi, int n, int T,
+                                                           int e, int32_t *out) {
     __shared__ WaveLds L;
+    __shared__ uint16_t colex[kColexPairs];
+    fill_colex(colex);
     const int lane = threadIdx.x;
     const int sw = spi_words_for(kMaxN);
     uint32_t w = 0;
@@ -698,11 +954,12 @@ __global__ __launch_bounds__(64) void eval_children_kernel(const uint8_t *pi, co
     }
     wave_sync();
     build_posinfo(L, n, sw, lane);
-    for (int t = lane; t < T; t += kWave) {
-        const uint32_t tp = tri[t];
-        out[t] = e + parity_delta(L.posinfo[tp & 0xFF], L.posinfo[(tp >> 8) & 0xFF],
-                                  L.posinfo[(tp >> 16) & 0xFF]);
-    }
+    // same enumeration and delta as the search scan, written at t = T-1-o
+    for_each_child_chunk(colex, n, T, lane, [&](bool valid, int i, int j, int k, int obase) {
+        const int delta = child_delta(L.posinfo[i], L.posinfo[j], L.posinfo[k]);
+        if (valid) out[T - 1 - (obase + lane)] = e + delta;
+        return false;
+    });
 }
 
 }  // namespace tdp

@@ -99,7 +99,9 @@ def filter_last_kernel_ms() -> float:
 def last_launch_stats(slot: int) -> dict:
     s = _lib.LaunchStats()
     _lib.check(_lib.load().tdp_last_launch_stats(slot, ctypes.byref(s)))
-    return {name: getattr(s, name) for name, _ in s._fields_}
+    out = {name: getattr(s, name) for name, _ in s._fields_ if name != "phase_ticks"}
+    out["phase_ms"] = {k: s.phase_ticks[q] * 1e-5 for q, k in enumerate(_lib.PHASES)}
+    return out
 
 
 @dataclass

@@ -28,8 +28,8 @@ def gpu_lib():
 
 @pytest.fixture(scope="module")
 def slots(gpu_lib):
-    """Two slots on device 0: 256 MiB of queue memory, 2^22 dedup entries."""
+    """Two slots on device 0: 2 GiB of queue memory, 2^22 dedup entries."""
     import tdp1375_amd
-    tdp1375_amd.init_slots(2, 1, 256 << 20, 1 << 22)
+    tdp1375_amd.init_slots(2, 1, 2 << 30, 1 << 22)
     yield [0, 1]
     tdp1375_amd.destroy_slots()

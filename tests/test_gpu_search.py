@@ -124,8 +124,17 @@ def test_parallel_single_searches_match_oracle(slots):
 def test_case_files_found(slots):
     """Every configuration of the reference's case files is FOUND at D = witness length."""
     data = load_cases()
-    for key, rate in (("3_2", MIN_RATE_15), ("oriented_7cycle", MIN_RATE_1375), ("comb", MIN_RATE_1375)):
-        entries = data[key] if key != "comb" else data[key][::7]
+    comb4 = [e for e in data["comb"] if len(e["witness"]) == 4]
+    comb_deep = [e for e in data["comb"] if len(e["witness"]) > 4][::12]
+    groups = (
+        (data["3_2"], MIN_RATE_15, (T.FLAG_DETERMINISTIC, 0)),
+        (data["oriented_7cycle"], MIN_RATE_1375, (T.FLAG_DETERMINISTIC, 0)),
+        (comb4, MIN_RATE_1375, (T.FLAG_DETERMINISTIC, 0)),
+        # (8,12) and (11,16) entries: the reference's depth-first order needs up
+        # to millions of expansions on one wave; the work-sharing mode is used
+        (comb_deep, MIN_RATE_1375, (0,)),
+    )
+    for entries, rate, modes in groups:
         pis, spis, e0s, ds = [], [], [], []
         for ent in entries:
             pi, spi = case_inputs(ent)
@@ -133,9 +142,9 @@ def test_case_files_found(slots):
             spis.append(spi)
             e0s.append(W.odd_cycle_count(spi))
             ds.append(len(ent["witness"]))
-        for flags in (T.FLAG_DETERMINISTIC, 0):
+        for flags in modes:
             got = T.search_batch(slots[1], pis, spis, e0s, rate, ds, 0, flags=flags)
-            assert got.found.all(), (key, np.nonzero(~got.found))
+            assert got.found.all(), (flags, np.nonzero(~got.found))
             for c in range(len(pis)):
                 assert_witness(pis[c], spis[c], e0s[c], rate, ds[c], got.moves[c])
 
