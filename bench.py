@@ -30,13 +30,13 @@ METRIC = "states expanded/s + cases/hour at depth 11, 1 MI355X; learned vs plain
 def algorithmic_bytes(stats: dict) -> int:
     """Queue and dedup traffic one launch has to move:
     node reads (pops + steals) and writes (pushes + donations, each donation
-    also reads the node), dedup tag words (8 B) and depth words (4 B) per probe,
-    stored keys compared (key bytes + 4 B configuration) and written
-    (key bytes + 4 B configuration + 4 B depth)."""
+    also reads the node); 8-byte dedup tags read (probes), stored keys
+    compared (key bytes plus the 8-byte tag re-read), and entries written
+    (key bytes, the tag CAS and the tag publication, 8 + 8 bytes)."""
     nb = int(stats["node_bytes"])
     kb = int(stats["key_bytes"])
     node = nb * (int(stats["pops"]) + int(stats["pushes"]) + 2 * int(stats["donations"]) + int(stats["steals"]))
-    dedup = 12 * int(stats["probes"]) + (kb + 4) * int(stats["key_reads"]) + (kb + 8) * int(stats["key_writes"])
+    dedup = 8 * int(stats["probes"]) + (kb + 8) * int(stats["key_reads"]) + (kb + 16) * int(stats["key_writes"])
     return node + dedup
 
 

@@ -73,6 +73,7 @@ typedef struct tdp_launch_stats {
     uint64_t key_reads;      /* dedup keys compared                          */
     uint64_t key_writes;     /* dedup entries written                        */
     uint64_t key_bytes;      /* bytes of one stored key                      */
+    uint64_t evictions;      /* dedup entries replaced while live (full line) */
     uint64_t overflow_info;  /* on overflow: reason << 56 (1 no sharing, 2 pool */
                              /* full, 3 stack still full) | stack fill << 32 |  */
                              /* pool fill                                      */
@@ -116,7 +117,8 @@ TDP_API int tdp_search_ex(int slot, const uint8_t *pi, const uint8_t *spi, int n
                   int32_t *out_moves, int32_t *out_depth, tdp_counters *out_counters);
 
 /* Many configurations in one persistent launch on one slot (extension; no
- * reference counterpart).  pis/spis: count x 64 bytes (first n[c] used). */
+ * reference counterpart).  pis/spis: count x 64 bytes (first n[c] used);
+ * count <= 65536. */
 TDP_API int tdp_search_batch(int slot, int count, const uint8_t *pis, const uint8_t *spis, const int32_t *ns,
                      const int32_t *initial_even_cycles, const float *min_rates,
                      const int32_t *max_moves, const int32_t *full_sorting, uint32_t flags,

@@ -59,7 +59,7 @@ class LaunchStats(ctypes.Structure):
                 ("donations", ctypes.c_uint64), ("steals", ctypes.c_uint64), ("node_bytes", ctypes.c_uint64),
                 ("pops", ctypes.c_uint64), ("pushes", ctypes.c_uint64), ("probes", ctypes.c_uint64),
                 ("key_reads", ctypes.c_uint64), ("key_writes", ctypes.c_uint64), ("key_bytes", ctypes.c_uint64),
-                ("overflow_info", ctypes.c_uint64), ("phase_ticks", ctypes.c_uint64 * 8)]
+                ("evictions", ctypes.c_uint64), ("overflow_info", ctypes.c_uint64), ("phase_ticks", ctypes.c_uint64 * 8)]
 
 PHASES = ("pop", "check", "analyse", "scan", "dedup", "push", "share", "root")
 

@@ -127,6 +127,17 @@ TDP_HD int node_words_for(int max_n) {
     return (w + 3) & ~3;  // 16-byte stride
 }
 
+// Dedup tag word: epoch << 48 | configuration << 32 | hash27 << 5 |
+// ready << 4 | depth.  An entry whose epoch is not the launch's is free; the
+// configuration field limits a launch to 65,536 configurations.
+constexpr int kDedupBucket = 8;
+constexpr unsigned long long kTagReady = 16ULL;
+TDP_HD unsigned long long dedup_ident(uint32_t epoch, uint32_t cfg, uint64_t hash) {
+    return ((unsigned long long)(epoch & 0xFFFFu) << 48) | ((unsigned long long)(cfg & 0xFFFFu) << 32) |
+           ((unsigned long long)(hash & 0x7FFFFFFu) << 5);
+}
+constexpr int kMaxBatchConfigs = 65536;
+
 // Launch-wide control block; hot words sit on their own 128-byte lines.
 struct alignas(128) ControlLine {
     unsigned long long v;
@@ -148,6 +159,7 @@ enum ControlIndex {
     kCtlProbes,        // dedup tag words read
     kCtlKeyReads,      // dedup entries whose stored key was compared
     kCtlKeyWrites,     // dedup entries written
+    kCtlEvictions,     // dedup entries replaced while live (bucket full)
     kCtlOverflowInfo,  // why the overflow flag was raised: reason << 56 | pool fill
     kCtlPhase0,        // per-phase time of profiling builds (kNumPhases lines)
     kNumControl = kCtlPhase0 + 8

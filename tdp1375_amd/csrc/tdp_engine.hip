@@ -299,6 +299,8 @@ struct Slot {
 
     int64_t queue_budget = 0;
     int64_t dedup_size = 0;
+    uint64_t dd_buckets = 0;   // dedup_size / dd_width lines of kDedupBucket tags
+    uint32_t dd_width = 0;     // tags used per line: min(8, dedup_size)
     uint32_t epoch = 0;
 
     DevPtr queue_mem;          // stacks + pool nodes
@@ -306,7 +308,7 @@ struct Slot {
     DevPtr pool_seq;
     size_t pool_seq_cap = 0;
     DevPtr ctrl;
-    DevPtr dd_tag, dd_best, dd_cfg, dd_key;
+    DevPtr dd_tag, dd_key;
 
     DevPtr cfgs, results;
     HostPtr h_cfgs, h_results, h_ctrl;
@@ -338,8 +340,6 @@ void destroy_slot(Slot &s) {
     s.pool_seq.reset();
     s.ctrl.reset();
     s.dd_tag.reset();
-    s.dd_best.reset();
-    s.dd_cfg.reset();
     s.dd_key.reset();
     s.cfgs.reset();
     s.results.reset();
@@ -396,13 +396,15 @@ void init_slots(int total, int devices, int64_t budget, int64_t dedup) {
         s->ctrl = make_dev(sizeof(ControlLine) * kNumControl, "sort.control");
         s->h_ctrl = make_pinned(sizeof(ControlLine) * kNumControl);
         if (s->dedup_size > 0) {
-            const size_t cap = (size_t)s->dedup_size;
-            s->dd_tag = make_dev(cap * 8, "sort.dedupTable");
-            s->dd_best = make_dev(cap * 4, "sort.dedupTable");
-            s->dd_cfg = make_dev(cap * 4, "sort.dedupTable");
-            s->dd_key = make_dev(cap * 64, "sort.dedupTable");
-            HIP_CHECK(hipMemset(s->dd_tag.get(), 0, cap * 8));
-            HIP_CHECK(hipMemset(s->dd_best.get(), 0, cap * 4));
+            // dedupTableSize entries as lines of 8 tags (a size that is not a
+            // multiple of 8 loses its last partial line; below 8 entries one
+            // line holds them all)
+            s->dd_width = (uint32_t)std::min<int64_t>(kDedupBucket, s->dedup_size);
+            s->dd_buckets = (uint64_t)s->dedup_size / s->dd_width;
+            const size_t lines = (size_t)s->dd_buckets;
+            s->dd_tag = make_dev(lines * kDedupBucket * 8, "sort.dedupTable");
+            s->dd_key = make_dev(lines * kDedupBucket * 64, "sort.dedupTable");
+            HIP_CHECK(hipMemset(s->dd_tag.get(), 0, lines * kDedupBucket * 8));
         }
         s->epoch = 0;
         g_slots.push_back(std::move(s));
@@ -515,8 +517,7 @@ void run_launch(Slot &s, const std::vector<CaseInput> &cases, uint32_t flags, bo
     if (s.dedup_size > 0) {
         s.epoch++;
         if (s.epoch > 0xFFFF) {
-            HIP_CHECK(hipMemsetAsync(s.dd_tag.get(), 0, (size_t)s.dedup_size * 8, s.stream));
-            HIP_CHECK(hipMemsetAsync(s.dd_best.get(), 0, (size_t)s.dedup_size * 4, s.stream));
+            HIP_CHECK(hipMemsetAsync(s.dd_tag.get(), 0, (size_t)s.dd_buckets * kDedupBucket * 8, s.stream));
             s.epoch = 1;
         }
     }
@@ -534,11 +535,10 @@ void run_launch(Slot &s, const std::vector<CaseInput> &cases, uint32_t flags, bo
     P.pool_seq = as<unsigned long long>(s.pool_seq);
     P.pool_cap = pool_cap ? pool_cap : 1;
     P.ctrl = as<ControlLine>(s.ctrl);
-    P.dd_cap = s.dedup_size > 0 ? (unsigned long long)s.dedup_size : 0ULL;
     P.dd_tag = as<unsigned long long>(s.dd_tag);
-    P.dd_best = as<uint32_t>(s.dd_best);
-    P.dd_cfg = as<uint32_t>(s.dd_cfg);
     P.dd_key = as<uint32_t>(s.dd_key);
+    P.dd_buckets = s.dd_buckets;
+    P.dd_width = s.dd_width;
     P.epoch = s.epoch ? s.epoch : 1;
     P.cancel_flag = as<int>(s.cancel_flag);
     P.donate = donate ? 1 : 0;
@@ -586,6 +586,7 @@ void run_launch(Slot &s, const std::vector<CaseInput> &cases, uint32_t flags, bo
     s.stats.probes = ctl[kCtlProbes].v;
     s.stats.key_reads = ctl[kCtlKeyReads].v;
     s.stats.key_writes = ctl[kCtlKeyWrites].v;
+    s.stats.evictions = ctl[kCtlEvictions].v;
     s.stats.key_bytes = (uint64_t)spi_words_for(max_n) * 4;
     s.stats.overflow_info = ctl[kCtlOverflowInfo].v;
     for (int k = 0; k < kNumPhases; ++k) s.stats.phase_ticks[k] = ctl[kCtlPhase0 + k].v;
@@ -839,6 +840,7 @@ int tdp_search_batch(int slot, int count, const uint8_t *pis, const uint8_t *spi
                      const int32_t *full_sorting, uint32_t flags, tdp_case_result *out) {
     return guarded([&] {
         if (count < 0) throw std::invalid_argument("negative batch size");
+        if (count > kMaxBatchConfigs) throw std::invalid_argument("batch larger than 65536 configurations");
         if (count == 0) return;
         if (!pis || !spis || !ns || !e0s || !min_rates || !max_moves || !full_sorting || !out)
             throw std::invalid_argument("null argument");

@@ -26,11 +26,12 @@ struct SearchParams {
 
     ControlLine *ctrl;            // kNumControl lines
 
-    unsigned long long *dd_tag;   // dd_cap
-    uint32_t *dd_best;            // dd_cap: epoch << 16 | depth
-    uint32_t *dd_cfg;             // dd_cap
-    uint32_t *dd_key;             // dd_cap x 16 words (spi bytes)
-    unsigned long long dd_cap;    // 0: no dedup table
+    // dedup table: dd_buckets lines of dd_width 64-bit tags (one 64-byte
+    // line per bucket) and one 64-byte key line (child spi words) per entry
+    unsigned long long *dd_tag;   // dd_buckets x dd_width
+    uint32_t *dd_key;             // dd_buckets x dd_width x 16 words
+    unsigned long long dd_buckets;  // 0: no dedup table
+    uint32_t dd_width;            // tags per bucket, 1..8
     uint32_t epoch;               // 1..65535
 
     const int *cancel_flag;

@@ -37,11 +37,9 @@ namespace {
 
 constexpr int kWavesPerBlock = 4;
 constexpr int kStage = 512;
-// Probe window of the dedup table: a child whose probe sequence finds neither
-// its key nor a free entry within this many slots is admitted without an
-// entry (the reference gives up after 8, sorting_search.cu:36,291-297); this
-// keeps a saturated table from stalling the search.
-constexpr int kMaxProbes = 64;
+// Rounds of one dedup lookup before the child is admitted without an entry
+// (a round is lost only to a concurrent update of the same bucket).
+constexpr int kMaxDedupRounds = 16;
 
 enum DedupOutcome { kDedupPending = -1, kDedupDominated = 0, kDedupImproved = 1,
                     kDedupInserted = 2, kDedupUninserted = 3 };
@@ -54,7 +52,8 @@ struct WaveLds {
     uint32_t stage[kStage];
     unsigned long long cnt[kNumCounters];    // counters of the current configuration
     unsigned long long pend[kNumCounters];   // counters of the current expansion
-    unsigned long long traffic[5];           // pops, pushes, tag reads, key compares, key writes
+    unsigned long long traffic[6];           // pops, pushes, tag reads, key compares, key writes,
+                                             // evictions
     unsigned long long phase[kNumPhases];    // 10 ns ticks per phase (TDP_PHASE_TIMING builds)
 };
 
@@ -461,10 +460,20 @@ __device__ void push_child(const SearchParams &P, WaveState &W, WaveLds &L, int 
     lds_add(&L.traffic[1], 1);
 }
 
-// Lane-parallel exact dedup (sorting_search.cu:261-298 semantics with an
-// exact table): returns a DedupOutcome for active lanes.
+// Lane-parallel dedup of staged children (sorting_search.cu:261-298
+// semantics: a child already stored at a depth <= its own is dominated, one
+// stored deeper is admitted and lowers the stored depth, an unknown one is
+// inserted).  Each lane reads its bucket's eight tags as independent loads
+// (one 64-byte line, one memory round trip), compares the stored key of every
+// ready tag with its identity (epoch, configuration, 27 hash bits), and
+// settles the depth with a CAS on the tag word.  A full bucket evicts an entry
+// of a configuration already found, else the deepest entry not shallower than
+// the child (its subtree is the cheapest to search again); a bucket of
+// shallower entries admits the child without an entry.  Losing an entry only
+// costs repeated work: a verdict never depends on what the table holds.
+// While no bucket fills the table is exact, as the CPU oracle's.
 struct LaneTraffic {
-    int probes, key_reads, key_writes;
+    int probes, key_reads, key_writes, evictions;
 };
 
 __device__ __forceinline__ int wave_sum(int v) {
@@ -475,7 +484,7 @@ __device__ __forceinline__ int wave_sum(int v) {
 
 __device__ int dedup_admit(const SearchParams &P, bool active, uint32_t cfg, const uint32_t *pw,
                            int nw, const ChildMove &m, int depth, LaneTraffic &tr) {
-    if (P.dd_cap == 0) return active ? kDedupUninserted : kDedupPending;
+    if (P.dd_buckets == 0) return active ? kDedupUninserted : kDedupDominated;
     uint64_t h = 0x9E3779B97F4A7C15ULL * (uint64_t)(cfg + 1);
     for (int q = 0; q < nw; ++q) {
         h ^= child_spi_word(pw, q, m);
@@ -483,50 +492,86 @@ __device__ int dedup_admit(const SearchParams &P, bool active, uint32_t cfg, con
         h ^= h >> 29;
     }
     h = fmix64(h);
+    const unsigned long long ident = dedup_ident(P.epoch, cfg, h);
+    const unsigned long long bucket = __umul64hi(h, P.dd_buckets);
+    unsigned long long *tags = P.dd_tag + bucket * kDedupBucket;
+    const int width = (int)P.dd_width;
     const unsigned long long epoch = P.epoch;
-    const unsigned long long mytag = (epoch << 48) | (h & 0xFFFFFFFFFFFFULL);
-    const uint32_t bword = ((uint32_t)epoch << 16) | (uint32_t)depth;
-    unsigned long long slot = __umul64hi(h, P.dd_cap);
-    int probes = 0;
     int res = active ? kDedupPending : kDedupDominated;
+    int rounds = 0;
     while (__ballot(res == kDedupPending) != 0) {
         if (res == kDedupPending) {
-            const unsigned long long tg = ld_relaxed(&P.dd_tag[slot]);
-            tr.probes++;
-            bool advance = false;
-            if ((tg >> 48) != epoch) {
-                if (cas64(&P.dd_tag[slot], tg, mytag)) {
-                    st32(&P.dd_cfg[slot], cfg);
-                    uint32_t *kw = P.dd_key + slot * 16;
-                    for (int q = 0; q < nw; ++q) st32(kw + q, child_spi_word(pw, q, m));
-                    complete_before();
-                    st32(&P.dd_best[slot], bword);
-                    res = kDedupInserted;
-                    tr.key_writes++;
-                }
-            } else if (tg == mytag) {
-                const uint32_t bw = ld32(&P.dd_best[slot]);
+            unsigned long long t[kDedupBucket];
+#pragma unroll
+            for (int q = 0; q < kDedupBucket; ++q) t[q] = q < width ? ld_relaxed(tags + q) : ~0ULL;
+            tr.probes += width;
+            bool retry = false, busy = false;
+            // 1. entries with this identity: compare keys
+            for (int q = 0; q < width && res == kDedupPending && !retry; ++q) {
+                if ((t[q] & ~31ULL) != ident) continue;
+                if (!(t[q] & kTagReady)) { busy = true; continue; }
+                const uint32_t *kw = P.dd_key + (bucket * kDedupBucket + q) * 16;
+                bool eq = true;
+                for (int w = 0; w < nw; ++w) eq &= ld32(kw + w) == child_spi_word(pw, w, m);
+                tr.key_reads++;
+                if (!eq) continue;
+                // the key read belongs to this entry only if the tag did not
+                // change meanwhile (an eviction rewrites the identity first)
                 complete_after();
-                if ((bw >> 16) == (uint32_t)epoch) {
-                    tr.key_reads++;
-                    bool eq = ld32(&P.dd_cfg[slot]) == cfg;
-                    const uint32_t *kw = P.dd_key + slot * 16;
-                    for (int q = 0; q < nw && eq; ++q) eq = ld32(kw + q) == child_spi_word(pw, q, m);
-                    if (eq) {
-                        const uint32_t old = __hip_atomic_fetch_min(&P.dd_best[slot], bword, __ATOMIC_RELAXED,
-                                                                    __HIP_MEMORY_SCOPE_AGENT);
-                        res = ((old & 0xFFFFu) <= (uint32_t)depth) ? kDedupDominated : kDedupImproved;
+                const unsigned long long cu = ld_relaxed(tags + q);
+                if ((cu & ~15ULL) != (t[q] & ~15ULL)) { retry = true; break; }
+                if ((int)(cu & 15ULL) <= depth) {
+                    res = kDedupDominated;
+                } else if (cas64(tags + q, cu, (cu & ~15ULL) | (unsigned long long)depth)) {
+                    res = kDedupImproved;
+                } else {
+                    retry = true;
+                }
+            }
+            // 2. not stored: insert into a free entry or evict
+            if (res == kDedupPending && !retry) {
+                if (busy) {
+                    retry = true;  // the same identity is being written
+                } else {
+                    int victim = -1;
+                    bool live = false;
+                    for (int q = 0; q < width && victim < 0; ++q)
+                        if ((t[q] >> 48) != epoch) victim = q;
+                    if (victim < 0) {
+                        uint32_t fnd[kDedupBucket];
+#pragma unroll
+                        for (int q = 0; q < kDedupBucket; ++q)
+                            fnd[q] = (q < width && (t[q] & kTagReady))
+                                         ? (uint32_t)__hip_atomic_load(&P.results[(t[q] >> 32) & 0xFFFFu].found,
+                                                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                                         : 0u;
+                        for (int q = 0; q < width && victim < 0; ++q)
+                            if (fnd[q]) victim = q;
+                        if (victim < 0) {
+                            int vd = depth - 1;
+                            for (int q = 0; q < width; ++q)
+                                if ((t[q] & kTagReady) && (int)(t[q] & 15ULL) > vd) { vd = (int)(t[q] & 15ULL); victim = q; }
+                            live = true;
+                        }
+                    }
+                    if (victim < 0) {
+                        res = kDedupUninserted;
+                    } else if (cas64(tags + victim, t[victim], ident | (unsigned long long)depth)) {
+                        // not ready: nobody else touches the entry until the
+                        // key is complete and the ready bit is published
+                        uint32_t *kw = P.dd_key + (bucket * kDedupBucket + victim) * 16;
+                        for (int w = 0; w < nw; ++w) st32(kw + w, child_spi_word(pw, w, m));
+                        complete_before();
+                        st64(tags + victim, ident | kTagReady | (unsigned long long)depth);
+                        res = kDedupInserted;
+                        tr.key_writes++;
+                        if (live) tr.evictions++;
                     } else {
-                        advance = true;
+                        retry = true;
                     }
                 }
-            } else {
-                advance = true;
             }
-            if (advance) {
-                slot = (slot + 1 == P.dd_cap) ? 0 : slot + 1;
-                if (++probes >= kMaxProbes) res = kDedupUninserted;
-            }
+            if (retry && ++rounds >= kMaxDedupRounds) res = kDedupUninserted;
         }
     }
     return res;
@@ -550,13 +595,14 @@ __device__ void flush_stage(const SearchParams &P, WaveState &W, WaveLds &L, int
         ChildMove m;
         m.a = pi[i]; m.b = pi[j]; m.c = pi[k];
         m.na = spi[m.c]; m.nb = spi[m.a]; m.nc = spi[m.b];
-        LaneTraffic tr{0, 0, 0};
+        LaneTraffic tr{0, 0, 0, 0};
         PHASE_T0();
         const int r = dedup_admit(P, act, cfg, L.node + kNodeHeaderWords, sw, m, d + 1, tr);
         PHASE_ADD(L, kPhaseDedup);
         lds_add(&L.traffic[2], (unsigned long long)wave_sum(tr.probes));
         lds_add(&L.traffic[3], (unsigned long long)wave_sum(tr.key_reads));
         lds_add(&L.traffic[4], (unsigned long long)wave_sum(tr.key_writes));
+        lds_add(&L.traffic[5], (unsigned long long)wave_sum(tr.evictions));
         const bool admit = act && r != kDedupDominated;
         if (!root_stage) {
             const unsigned long long dom = __popcll(__ballot(act && r == kDedupDominated));
@@ -748,7 +794,7 @@ __global__ __launch_bounds__(256) void search_kernel(SearchParams P) {
     W.stop = false;
     W.deadline = P.timeout_ticks ? __builtin_amdgcn_s_memrealtime() + P.timeout_ticks : 0ULL;
     if (W.lane < kNumCounters) L.cnt[W.lane] = 0;
-    if (W.lane < 5) L.traffic[W.lane] = 0;
+    if (W.lane < 6) L.traffic[W.lane] = 0;
     if (W.lane < kNumPhases) L.phase[W.lane] = 0;
     wave_sync();
     if (W.lane == 0) fetch_add(&P.ctrl[kCtlRegistered].v, 1ULL);
@@ -911,6 +957,7 @@ __global__ __launch_bounds__(256) void search_kernel(SearchParams P) {
         fetch_add(&P.ctrl[kCtlProbes].v, L.traffic[2]);
         fetch_add(&P.ctrl[kCtlKeyReads].v, L.traffic[3]);
         fetch_add(&P.ctrl[kCtlKeyWrites].v, L.traffic[4]);
+        fetch_add(&P.ctrl[kCtlEvictions].v, L.traffic[5]);
     }
 }
 
