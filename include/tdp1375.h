@@ -64,8 +64,8 @@ typedef struct tdp_launch_stats {
     double kernel_ms;        /* search kernel time (HIP events)              */
     int32_t waves;           /* waves launched                               */
     int32_t overflowed;      /* queue overflow flag                          */
-    uint64_t donations;      /* nodes moved to the shared pool               */
-    uint64_t steals;         /* nodes taken from the shared pool             */
+    uint64_t donations;      /* nodes handed to idle waves                   */
+    uint64_t steals;         /* nodes received by idle waves                 */
     uint64_t node_bytes;     /* bytes per node                               */
     uint64_t pops;           /* nodes read from a wave's stack               */
     uint64_t pushes;         /* nodes written to a wave's stack              */
@@ -74,9 +74,10 @@ typedef struct tdp_launch_stats {
     uint64_t key_writes;     /* dedup entries written                        */
     uint64_t key_bytes;      /* bytes of one stored key                      */
     uint64_t evictions;      /* dedup entries replaced while live (full line) */
+    uint64_t spills;         /* nodes moved from full stacks to the spill pool */
+    uint64_t spill_takes;    /* nodes taken from the spill pool              */
     uint64_t overflow_info;  /* on overflow: reason << 56 (1 no sharing, 2 pool */
-                             /* full, 3 stack still full) | stack fill << 32 |  */
-                             /* pool fill                                      */
+                             /* full) | stack fill << 32 | pool fill           */
     uint64_t phase_ticks[8]; /* profiling builds only: 10 ns ticks summed over */
                              /* waves for pop, checks, cycle analysis, scan,  */
                              /* dedup, push, sharing, root expansion          */
@@ -153,6 +154,13 @@ TDP_API int tdp_eval_children(const uint8_t *pi, const uint8_t *spi, int n, int3
 TDP_API int tdp_host_eval_children(const uint8_t *pi, const uint8_t *spi, int n, int32_t *out);
 TDP_API int tdp_host_predicate_tables(int n, int initial_even_cycles, float min_rate, int max_depth,
                               int full_sorting, uint64_t *accept16, uint64_t *reach16);
+
+/* Work-sharing launch tuning for later launches on every slot (extension;
+ * no reference counterpart).  0 keeps a value's default: waves per CU = the
+ * kernel's occupancy limit, donation check every 16 expansions, 4 nodes per
+ * donation, idle sleep up to 16 x 127 s_sleep units.  share_interval must be
+ * a power of two. */
+TDP_API int tdp_set_tuning(int waves_per_cu, int share_interval, int share_max, int idle_backoff);
 
 /* Number of visible GPUs (0 when none). */
 TDP_API int tdp_device_count(void);

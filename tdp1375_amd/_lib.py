@@ -31,7 +31,7 @@ EXPORTED_SYMBOLS = (
     "tdp_last_error", "tdp_init_slots", "tdp_destroy_slots", "tdp_search", "tdp_search_ex",
     "tdp_search_batch", "tdp_last_launch_stats", "tdp_cancel", "tdp_filter",
     "tdp_filter_last_kernel_ms", "tdp_eval_children", "tdp_host_eval_children",
-    "tdp_host_predicate_tables", "tdp_device_count",
+    "tdp_host_predicate_tables", "tdp_set_tuning", "tdp_device_count",
     "Java_br_unb_cic_tdp_base_GPUSortingSearch_initSlots",
     "Java_br_unb_cic_tdp_base_GPUSortingSearch_destroySlots",
     "Java_br_unb_cic_tdp_base_GPUSortingSearch_searchForSortingSeq",
@@ -59,7 +59,8 @@ class LaunchStats(ctypes.Structure):
                 ("donations", ctypes.c_uint64), ("steals", ctypes.c_uint64), ("node_bytes", ctypes.c_uint64),
                 ("pops", ctypes.c_uint64), ("pushes", ctypes.c_uint64), ("probes", ctypes.c_uint64),
                 ("key_reads", ctypes.c_uint64), ("key_writes", ctypes.c_uint64), ("key_bytes", ctypes.c_uint64),
-                ("evictions", ctypes.c_uint64), ("overflow_info", ctypes.c_uint64), ("phase_ticks", ctypes.c_uint64 * 8)]
+                ("evictions", ctypes.c_uint64), ("spills", ctypes.c_uint64),
+                ("spill_takes", ctypes.c_uint64), ("overflow_info", ctypes.c_uint64), ("phase_ticks", ctypes.c_uint64 * 8)]
 
 PHASES = ("pop", "check", "analyse", "scan", "dedup", "push", "share", "root")
 
@@ -118,6 +119,7 @@ def load() -> ctypes.CDLL:
         L.tdp_host_predicate_tables.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_int,
                                                 ctypes.c_int, ctypes.POINTER(ctypes.c_uint64),
                                                 ctypes.POINTER(ctypes.c_uint64)]
+        L.tdp_set_tuning.argtypes = [ctypes.c_int] * 4
         L.tdp_device_count.argtypes = []
         _lib = L
         return L

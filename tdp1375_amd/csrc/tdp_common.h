@@ -147,9 +147,9 @@ enum ControlIndex {
     kCtlNextConfig = 0,
     kCtlIdle,
     kCtlRegistered,
-    kCtlPoolHead,
-    kCtlPoolTail,
-    kCtlReserved5,     // unused
+    kCtlPoolHead,      // spill pool: nodes taken
+    kCtlPoolTail,      // spill pool: slots reserved
+    kCtlSpills,        // nodes moved to the spill pool
     kCtlStop,          // 1 overflow
     kCtlResolved,      // configurations found
     kCtlDonations,
@@ -160,7 +160,8 @@ enum ControlIndex {
     kCtlKeyReads,      // dedup entries whose stored key was compared
     kCtlKeyWrites,     // dedup entries written
     kCtlEvictions,     // dedup entries replaced while live (bucket full)
-    kCtlOverflowInfo,  // why the overflow flag was raised: reason << 56 | pool fill
+    kCtlSpillTakes,    // nodes taken from the spill pool
+    kCtlOverflowInfo,  // why the overflow flag was raised: reason << 56 | stack fill << 32 | pool fill
     kCtlPhase0,        // per-phase time of profiling builds (kNumPhases lines)
     kNumControl = kCtlPhase0 + 8
 };
@@ -173,7 +174,7 @@ enum PhaseIndex {
     kPhaseScan,         // child evaluation over all triples
     kPhaseDedup,        // dedup of staged children
     kPhasePush,         // pushes of admitted children
-    kPhaseShare,        // donation, idle polling and pool takes
+    kPhaseShare,        // donation, idle polling, inbox receipt and pool takes
     kPhaseRoot,         // root expansions
     kNumPhases
 };

@@ -281,6 +281,21 @@ DeviceShared &device_shared(int device) {
     return *g_dev[device];
 }
 
+// Launch tuning of the work-sharing mode (tdp_set_tuning); 0 = default.
+struct Tuning {
+    int waves_per_cu = 0;     // default: the occupancy limit of search_kernel
+    int share_interval = 16;  // expansions between donation checks (power of two)
+    int share_max = 4;        // nodes per donation
+    int idle_backoff = 16;    // longest idle sleep, x 127 s_sleep units
+};
+std::mutex g_tuning_mutex;
+Tuning g_tuning;
+
+Tuning current_tuning() {
+    std::lock_guard<std::mutex> lk(g_tuning_mutex);
+    return g_tuning;
+}
+
 // ---------------------------------------------------------------- slots
 struct CaseInput {
     const uint8_t *pi, *spi;
@@ -303,10 +318,12 @@ struct Slot {
     uint32_t dd_width = 0;     // tags used per line: min(8, dedup_size)
     uint32_t epoch = 0;
 
-    DevPtr queue_mem;          // stacks + pool nodes
+    DevPtr queue_mem;          // stacks + inbox nodes + spill pool nodes
     size_t queue_bytes = 0;
     DevPtr pool_seq;
     size_t pool_seq_cap = 0;
+    DevPtr share_lines;        // inbox lines + idle bits (parallel mode)
+    size_t share_lines_cap = 0;
     DevPtr ctrl;
     DevPtr dd_tag, dd_key;
 
@@ -337,6 +354,7 @@ void destroy_slot(Slot &s) {
     if (s.stream) (void)hipStreamSynchronize(s.stream);
     if (s.cancel_stream) (void)hipStreamSynchronize(s.cancel_stream);
     s.queue_mem.reset();
+    s.share_lines.reset();
     s.pool_seq.reset();
     s.ctrl.reset();
     s.dd_tag.reset();
@@ -492,9 +510,23 @@ void run_launch(Slot &s, const std::vector<CaseInput> &cases, uint32_t flags, bo
         waves = dev.waves_max;
         donate = true;
     }
-    // queue memory: 3/4 private stacks, 1/4 shared pool (sorting_search.cu:655-658)
-    size_t pool_bytes = donate ? s.queue_bytes / 4 : 0;
-    size_t stack_bytes = s.queue_bytes - pool_bytes;
+    // queue memory (queueBytesBudget, sorting_search.cu:655-658): the
+    // parallel mode sets aside one inbox of share_max nodes per wave and a
+    // quarter of the rest for the spill pool; the remainder is split evenly
+    // into private stacks (powers of two)
+    const Tuning tune = current_tuning();
+    if (!(flags & TDP_FLAG_SINGLE_WAVE) && tune.waves_per_cu > 0)
+        waves = std::min(waves, std::max(1, tune.waves_per_cu * dev.cus));
+    uint32_t share_max = donate ? (uint32_t)tune.share_max : 0;
+    size_t inbox_bytes = (size_t)waves * share_max * node_bytes;
+    if (donate && inbox_bytes * 4 > s.queue_bytes) {
+        // a budget this small runs without sharing
+        donate = false;
+        share_max = 0;
+        inbox_bytes = 0;
+    }
+    const size_t pool_bytes = donate ? (s.queue_bytes - inbox_bytes) / 4 : 0;
+    const size_t stack_bytes = s.queue_bytes - inbox_bytes - pool_bytes;
     uint32_t stack_cap = floor_pow2(stack_bytes / ((size_t)waves * node_bytes));
     if (stack_cap < 2) {
         // the budget cannot hold two nodes per wave: fewer waves
@@ -502,15 +534,19 @@ void run_launch(Slot &s, const std::vector<CaseInput> &cases, uint32_t flags, bo
         stack_cap = floor_pow2(stack_bytes / ((size_t)std::max(1, waves) * node_bytes));
     }
     if (stack_cap < 2) throw QueueOverflow("GPU sorting search queue overflow");
+    if (share_max > stack_cap / 2) share_max = stack_cap / 2;
+    if (share_max == 0) donate = false;
     uint32_t pool_cap = donate ? floor_pow2(pool_bytes / node_bytes) : 0;
-    if (donate && pool_cap < 64) {
-        donate = false;
-        pool_cap = 0;
+    if (pool_cap < 64) pool_cap = 0;
+    if (pool_cap && s.pool_seq_cap < pool_cap) {
+        s.pool_seq = make_dev((size_t)pool_cap * 8, "sort.poolSeq");
+        s.pool_seq_cap = pool_cap;
     }
-    const size_t pool_cap_alloc = std::max<size_t>(pool_cap, 1);
-    if (s.pool_seq_cap < pool_cap_alloc) {
-        s.pool_seq = make_dev(pool_cap_alloc * 8, "sort.poolSeq");
-        s.pool_seq_cap = pool_cap_alloc;
+    const uint32_t idle_words = (uint32_t)((waves + 63) / 64);
+    const size_t lines_needed = ((size_t)waves + idle_words) * 128;
+    if (donate && s.share_lines_cap < lines_needed) {
+        s.share_lines = make_dev(lines_needed, "sort.shareLines");
+        s.share_lines_cap = lines_needed;
     }
 
     // dedup epoch (replaces the per-search table memset, sorting_search.cu:744-747)
@@ -531,8 +567,14 @@ void run_launch(Slot &s, const std::vector<CaseInput> &cases, uint32_t flags, bo
     P.stack_cap = stack_cap;
     P.node_words = nw;
     P.spi_words = (uint32_t)spi_words_for(max_n);
-    P.pool_nodes = P.stacks + (size_t)waves * stack_cap * nw;
-    P.pool_seq = as<unsigned long long>(s.pool_seq);
+    P.inbox_nodes = donate ? P.stacks + (size_t)waves * stack_cap * nw : nullptr;
+    P.inbox = donate ? as<unsigned long long>(s.share_lines) : nullptr;
+    P.idle_bits = donate ? P.inbox + (size_t)waves * 16 : nullptr;
+    P.idle_words = donate ? idle_words : 0;
+    // pool after the stacks and inboxes; pool_cap 1 with no pool: every
+    // spill attempt finds it full
+    P.pool_nodes = P.stacks + (size_t)waves * (stack_cap + (donate ? share_max : 0)) * nw;
+    P.pool_seq = pool_cap ? as<unsigned long long>(s.pool_seq) : nullptr;
     P.pool_cap = pool_cap ? pool_cap : 1;
     P.ctrl = as<ControlLine>(s.ctrl);
     P.dd_tag = as<unsigned long long>(s.dd_tag);
@@ -542,12 +584,16 @@ void run_launch(Slot &s, const std::vector<CaseInput> &cases, uint32_t flags, bo
     P.epoch = s.epoch ? s.epoch : 1;
     P.cancel_flag = as<int>(s.cancel_flag);
     P.donate = donate ? 1 : 0;
+    P.share_mask = (uint32_t)tune.share_interval - 1;
+    P.share_max = share_max;
+    P.idle_backoff = (uint32_t)tune.idle_backoff;
     P.timeout_ticks = kernel_timeout_ticks();
 
     if (!resident)
         HIP_CHECK(hipMemcpyAsync(s.cfgs.get(), s.h_cfgs.get(), (size_t)count * sizeof(ConfigDesc),
                                  hipMemcpyHostToDevice, s.stream));
-    const size_t reset_n = std::max<size_t>({(size_t)count, (size_t)P.pool_cap, (size_t)kNumControl});
+    // one thread per configuration, wave (inbox line), pool slot and control line
+    const size_t reset_n = std::max<size_t>({(size_t)count, (size_t)waves, (size_t)pool_cap, (size_t)kNumControl});
     hipLaunchKernelGGL(reset_kernel, dim3((unsigned)((reset_n + 255) / 256)), dim3(256), 0, s.stream, P);
     HIP_CHECK(hipGetLastError());
     HIP_CHECK(hipEventRecord(s.ev0, s.stream));
@@ -587,6 +633,8 @@ void run_launch(Slot &s, const std::vector<CaseInput> &cases, uint32_t flags, bo
     s.stats.key_reads = ctl[kCtlKeyReads].v;
     s.stats.key_writes = ctl[kCtlKeyWrites].v;
     s.stats.evictions = ctl[kCtlEvictions].v;
+    s.stats.spills = ctl[kCtlSpills].v;
+    s.stats.spill_takes = ctl[kCtlSpillTakes].v;
     s.stats.key_bytes = (uint64_t)spi_words_for(max_n) * 4;
     s.stats.overflow_info = ctl[kCtlOverflowInfo].v;
     for (int k = 0; k < kNumPhases; ++k) s.stats.phase_ticks[k] = ctl[kCtlPhase0 + k].v;
@@ -804,6 +852,22 @@ using namespace tdp;
 extern "C" {
 
 const char *tdp_last_error(void) { return g_last_error.c_str(); }
+
+int tdp_set_tuning(int waves_per_cu, int share_interval, int share_max, int idle_backoff) {
+    return guarded([&] {
+        if (waves_per_cu < 0 || share_interval < 0 || share_max < 0 || idle_backoff < 0)
+            throw std::invalid_argument("tuning values must be non-negative");
+        if (share_interval && (share_interval & (share_interval - 1)))
+            throw std::invalid_argument("share_interval must be a power of two");
+        Tuning t;
+        if (waves_per_cu) t.waves_per_cu = waves_per_cu;
+        if (share_interval) t.share_interval = share_interval;
+        if (share_max) t.share_max = share_max;
+        if (idle_backoff) t.idle_backoff = idle_backoff;
+        std::lock_guard<std::mutex> lk(g_tuning_mutex);
+        g_tuning = t;
+    });
+}
 
 int tdp_device_count(void) {
     int c = 0;

@@ -20,6 +20,14 @@ struct SearchParams {
     uint32_t node_words;
     uint32_t spi_words;
 
+    // work sharing (parallel mode): per wave an inbox of share_max nodes and
+    // an inbox line (count of published nodes, 128-byte stride); one idle bit
+    // per wave in idle_words words (128-byte stride)
+    uint32_t *inbox_nodes;        // num_waves x share_max x node_words
+    unsigned long long *inbox;    // num_waves x 16
+    unsigned long long *idle_bits;  // idle_words x 16
+    uint32_t idle_words;
+    // spill pool for full stacks when no wave is idle
     uint32_t *pool_nodes;         // pool_cap x node_words
     unsigned long long *pool_seq; // pool_cap
     unsigned long long pool_cap;  // power of two
@@ -36,6 +44,9 @@ struct SearchParams {
 
     const int *cancel_flag;
     int donate;                   // 0: every configuration stays on one wave
+    uint32_t share_mask;          // donation check every share_mask + 1 expansions
+    uint32_t share_max;           // nodes handed over per donation
+    uint32_t idle_backoff;        // longest idle sleep, in units of s_sleep 127
     unsigned long long timeout_ticks;  // watchdog: 100 MHz ticks per wave, 0 = none
 };
 

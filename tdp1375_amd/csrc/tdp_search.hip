@@ -50,6 +50,7 @@ struct WaveLds {
     uint32_t node[48];
     uint32_t posinfo[64];
     uint32_t stage[kStage];
+    uint32_t ckey[16][kWave];                // child spi words of each lane's staged child
     unsigned long long cnt[kNumCounters];    // counters of the current configuration
     unsigned long long pend[kNumCounters];   // counters of the current expansion
     unsigned long long traffic[6];           // pops, pushes, tag reads, key compares, key writes,
@@ -315,9 +316,98 @@ __device__ void set_overflow(const SearchParams &P, WaveState &W) {
     W.stop = true;
 }
 
-// Moves the oldest node of the private stack into the shared pool.  Returns
+// ---- work sharing between waves (parallel mode)
+// A wave without work counts itself idle, sets its bit in idle_bits and then
+// polls only its own inbox line.  A busy wave that sees a set bit claims that
+// wave (fetch_and on the bit; the claim also takes the wave out of the idle
+// count), copies some of its oldest nodes (the largest subtrees) into the
+// claimed wave's inbox and publishes their number on the inbox line.  So the
+// idle waves, which are most of the grid for long stretches, never share a
+// polled line, and the launch is over exactly when every registered wave is
+// idle: work only moves from a busy wave, which is not counted idle.
+
+__device__ __forceinline__ unsigned long long *inbox_line(const SearchParams &P, int wave) {
+    return &P.inbox[(size_t)wave * 16];
+}
+__device__ __forceinline__ unsigned long long *idle_word(const SearchParams &P, int word) {
+    return &P.idle_bits[(size_t)word * 16];
+}
+
+// Claims one idle wave; returns its index or -1 (wave-uniform).
+__device__ int claim_idle_wave(const SearchParams &P, const WaveState &W, unsigned salt) {
+    int r = -1;
+    if (W.lane == 0) {
+        const int words = (int)P.idle_words;
+        const int tries = words < 4 ? words : 4;
+        int word = (int)(((unsigned)(W.wave >> 6) + salt) % (unsigned)words);
+        for (int t = 0; t < tries && r < 0; ++t) {
+            unsigned long long bits = ld_relaxed(idle_word(P, word));
+            while (bits && r < 0) {
+                const int b = __ffsll((long long)bits) - 1;
+                const unsigned long long bit = 1ULL << b;
+                const unsigned long long old =
+                    __hip_atomic_fetch_and(idle_word(P, word), ~bit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (old & bit) r = word * 64 + b;
+                else bits = old & ~bit;
+            }
+            word = (word + 1 == words) ? 0 : word + 1;
+        }
+        if (r >= 0) fetch_add(&P.ctrl[kCtlIdle].v, ~0ULL);
+    }
+    return uniform(__shfl(r, 0));
+}
+
+// Hands up to `most` of the oldest nodes of this wave's stack to an idle wave.
+// Returns the number handed over (0: no idle wave).
+__device__ uint32_t share_nodes(const SearchParams &P, WaveState &W, uint32_t most, unsigned salt) {
+    if (most == 0) return 0;
+    const int r = claim_idle_wave(P, W, salt);
+    if (r < 0) return 0;
+    const int nw = (int)P.node_words;
+    uint32_t *dst = P.inbox_nodes + (size_t)r * P.share_max * nw;
+    for (uint32_t g = 0; g < most; ++g) {
+        const uint32_t *src = stack_slot(P, W, W.lo + g);
+        if (W.lane < nw) st32(dst + g * nw + W.lane, src[W.lane]);
+    }
+    W.lo += most;
+    complete_before();
+    if (W.lane == 0) {
+        st64(inbox_line(P, r), (unsigned long long)most);
+        fetch_add(&P.ctrl[kCtlDonations].v, (unsigned long long)most);
+    }
+    return most;
+}
+
+// Moves the nodes published in this wave's inbox onto its stack (which is
+// empty: the wave was idle).  Returns false when the inbox is empty.
+__device__ bool receive_nodes(const SearchParams &P, WaveState &W) {
+    unsigned long long cnt = 0;
+    if (W.lane == 0) cnt = ld_relaxed(inbox_line(P, W.wave));
+    const int k = uniform(__shfl((int)cnt, 0));
+    if (k == 0) return false;
+    complete_after();
+    const int nw = (int)P.node_words;
+    const uint32_t *src = P.inbox_nodes + (size_t)W.wave * P.share_max * nw;
+    for (int g = 0; g < k; ++g) {
+        uint32_t *dst = stack_slot(P, W, W.hi);
+        if (W.lane < nw) dst[W.lane] = ld32(src + g * nw + W.lane);
+        W.hi++;
+    }
+    complete_before();
+    if (W.lane == 0) {
+        st64(inbox_line(P, W.wave), 0ULL);
+        fetch_add(&P.ctrl[kCtlSteals].v, (unsigned long long)k);
+    }
+    return true;
+}
+
+// ---- spill pool: a bounded MPMC ring (per-slot sequence numbers) that takes
+// the oldest nodes of a full stack when no wave is idle; idle waves drain it.
+
+// Moves the oldest node of the private stack into the spill pool.  Returns
 // false when the pool is full.
 __device__ bool donate_one(const SearchParams &P, WaveState &W) {
+    if (!P.pool_seq) return false;
     unsigned long long ticket = 0;
     int ok = 0;
     if (W.lane == 0) {
@@ -354,7 +444,10 @@ __device__ bool donate_one(const SearchParams &P, WaveState &W) {
     uint32_t *dst = P.pool_nodes + slot * P.node_words;
     if (W.lane < (int)P.node_words) st32(dst + W.lane, src[W.lane]);
     complete_before();
-    if (W.lane == 0) st64(&P.pool_seq[slot], ticket + 1);
+    if (W.lane == 0) {
+        st64(&P.pool_seq[slot], ticket + 1);
+        fetch_add(&P.ctrl[kCtlSpills].v, 1ULL);
+    }
     W.lo++;
     return true;
 }
@@ -391,23 +484,27 @@ __device__ bool take_one(const SearchParams &P, WaveState &W, WaveLds &L) {
     complete_before();
     if (W.lane == 0) {
         st64(&P.pool_seq[slot], ticket + P.pool_cap);
-        fetch_add(&P.ctrl[kCtlSteals].v, 1ULL);
+        fetch_add(&P.ctrl[kCtlSpillTakes].v, 1ULL);
     }
     wave_sync();
     return true;
 }
 
-// Ensures room for one push on the private stack.
+// Ensures room for one push on the private stack: a full stack hands its
+// oldest nodes to an idle wave, else spills them to the pool, else the launch
+// stops with a queue overflow.
 __device__ bool make_room(const SearchParams &P, WaveState &W) {
     if (W.hi - W.lo < P.stack_cap) return true;
-    unsigned long long reason = 1;   // 1: no sharing, 2: pool full, 3: stack still full
+    unsigned long long reason = 1;   // 1: no sharing, 2: pool full
     if (P.donate) {
-        int r = 0;
-        for (; r < 32 && W.hi - W.lo > 1; ++r)
+        uint32_t most = (W.hi - W.lo) / 2;
+        if (most > P.share_max) most = P.share_max;
+        if (share_nodes(P, W, most, W.hi)) return true;
+        for (int r = 0; r < 32 && W.hi - W.lo > 1; ++r)
             if (!donate_one(P, W)) break;
         if (W.stop) return false;
         if (W.hi - W.lo < P.stack_cap) return true;
-        reason = r < 32 ? 2 : 3;
+        reason = 2;
     }
     if (W.lane == 0) {
         const unsigned long long h = ctl_load(P, kCtlPoolHead);
@@ -482,12 +579,23 @@ __device__ __forceinline__ int wave_sum(int v) {
     return v;
 }
 
-__device__ int dedup_admit(const SearchParams &P, bool active, uint32_t cfg, const uint32_t *pw,
-                           int nw, const ChildMove &m, int depth, LaneTraffic &tr) {
+// t[q] for a wave-varying q without dynamic register indexing (which would
+// put the array in scratch memory)
+__device__ __forceinline__ unsigned long long pick_tag(const unsigned long long (&t)[kDedupBucket], int q) {
+    unsigned long long v = 0;
+#pragma unroll
+    for (int i = 0; i < kDedupBucket; ++i) v = (q == i) ? t[i] : v;
+    return v;
+}
+
+// The child's key words are in L.ckey[0..nw)[lane] (written by the caller),
+// so they do not occupy registers across the probe loop.
+__device__ __forceinline__ int dedup_admit(const SearchParams &P, WaveLds &L, int lane, bool active, uint32_t cfg,
+                                           int nw, int depth, LaneTraffic &tr) {
     if (P.dd_buckets == 0) return active ? kDedupUninserted : kDedupDominated;
     uint64_t h = 0x9E3779B97F4A7C15ULL * (uint64_t)(cfg + 1);
     for (int q = 0; q < nw; ++q) {
-        h ^= child_spi_word(pw, q, m);
+        h ^= L.ckey[q][lane];
         h *= 0xff51afd7ed558ccdULL;
         h ^= h >> 29;
     }
@@ -495,6 +603,7 @@ __device__ int dedup_admit(const SearchParams &P, bool active, uint32_t cfg, con
     const unsigned long long ident = dedup_ident(P.epoch, cfg, h);
     const unsigned long long bucket = __umul64hi(h, P.dd_buckets);
     unsigned long long *tags = P.dd_tag + bucket * kDedupBucket;
+    const uint32_t *keys = P.dd_key + bucket * kDedupBucket * 16;
     const int width = (int)P.dd_width;
     const unsigned long long epoch = P.epoch;
     int res = active ? kDedupPending : kDedupDominated;
@@ -503,24 +612,35 @@ __device__ int dedup_admit(const SearchParams &P, bool active, uint32_t cfg, con
         if (res == kDedupPending) {
             unsigned long long t[kDedupBucket];
 #pragma unroll
-            for (int q = 0; q < kDedupBucket; ++q) t[q] = q < width ? ld_relaxed(tags + q) : ~0ULL;
+            for (int q = 0; q < kDedupBucket; ++q) t[q] = q < width ? ld_relaxed(tags + q) : 0ULL;
             tr.probes += width;
             bool retry = false, busy = false;
-            // 1. entries with this identity: compare keys
-            for (int q = 0; q < width && res == kDedupPending && !retry; ++q) {
-                if ((t[q] & ~31ULL) != ident) continue;
-                if (!(t[q] & kTagReady)) { busy = true; continue; }
-                const uint32_t *kw = P.dd_key + (bucket * kDedupBucket + q) * 16;
+            uint32_t cand = 0;   // ready entries with this identity
+#pragma unroll
+            for (int q = 0; q < kDedupBucket; ++q) {
+                const bool same = q < width && (t[q] & ~31ULL) == ident;
+                busy |= same && !(t[q] & kTagReady);
+                cand |= (same && (t[q] & kTagReady)) ? (1u << q) : 0u;
+            }
+            // 1. compare the stored keys of the candidates
+#pragma unroll 1
+            while (cand && res == kDedupPending && !retry) {
+                const int q = __ffs(cand) - 1;
+                cand &= cand - 1;
+                const unsigned long long tq = pick_tag(t, q);
+                const uint32_t *kw = keys + q * 16;
                 bool eq = true;
-                for (int w = 0; w < nw; ++w) eq &= ld32(kw + w) == child_spi_word(pw, w, m);
+#pragma unroll 1
+                for (int w = 0; w < nw; ++w) eq &= ld32(kw + w) == L.ckey[w][lane];
                 tr.key_reads++;
                 if (!eq) continue;
                 // the key read belongs to this entry only if the tag did not
                 // change meanwhile (an eviction rewrites the identity first)
                 complete_after();
                 const unsigned long long cu = ld_relaxed(tags + q);
-                if ((cu & ~15ULL) != (t[q] & ~15ULL)) { retry = true; break; }
-                if ((int)(cu & 15ULL) <= depth) {
+                if ((cu & ~15ULL) != (tq & ~15ULL)) {
+                    retry = true;
+                } else if ((int)(cu & 15ULL) <= depth) {
                     res = kDedupDominated;
                 } else if (cas64(tags + q, cu, (cu & ~15ULL) | (unsigned long long)depth)) {
                     res = kDedupImproved;
@@ -534,33 +654,43 @@ __device__ int dedup_admit(const SearchParams &P, bool active, uint32_t cfg, con
                     retry = true;  // the same identity is being written
                 } else {
                     int victim = -1;
-                    bool live = false;
-                    for (int q = 0; q < width && victim < 0; ++q)
-                        if ((t[q] >> 48) != epoch) victim = q;
-                    if (victim < 0) {
-                        uint32_t fnd[kDedupBucket];
 #pragma unroll
-                        for (int q = 0; q < kDedupBucket; ++q)
-                            fnd[q] = (q < width && (t[q] & kTagReady))
-                                         ? (uint32_t)__hip_atomic_load(&P.results[(t[q] >> 32) & 0xFFFFu].found,
-                                                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
-                                         : 0u;
-                        for (int q = 0; q < width && victim < 0; ++q)
-                            if (fnd[q]) victim = q;
-                        if (victim < 0) {
+                    for (int q = kDedupBucket - 1; q >= 0; --q)
+                        if (q < width && (t[q] >> 48) != epoch) victim = q;
+                    bool live = false;
+                    if (victim < 0) {
+                        // an entry of a configuration already found
+                        uint32_t found_mask = 0;
+#pragma unroll
+                        for (int q = 0; q < kDedupBucket; ++q) {
+                            if (q < width && (t[q] & kTagReady)) {
+                                const uint32_t f = (uint32_t)__hip_atomic_load(
+                                    &P.results[(t[q] >> 32) & 0xFFFFu].found, __ATOMIC_RELAXED,
+                                    __HIP_MEMORY_SCOPE_AGENT);
+                                found_mask |= f ? (1u << q) : 0u;
+                            }
+                        }
+                        if (found_mask) {
+                            victim = __ffs(found_mask) - 1;
+                        } else {
+                            // the deepest entry not shallower than the child
                             int vd = depth - 1;
-                            for (int q = 0; q < width; ++q)
-                                if ((t[q] & kTagReady) && (int)(t[q] & 15ULL) > vd) { vd = (int)(t[q] & 15ULL); victim = q; }
+#pragma unroll
+                            for (int q = 0; q < kDedupBucket; ++q) {
+                                const int dq = (int)(t[q] & 15ULL);
+                                if (q < width && (t[q] & kTagReady) && dq > vd) { vd = dq; victim = q; }
+                            }
                             live = true;
                         }
                     }
                     if (victim < 0) {
                         res = kDedupUninserted;
-                    } else if (cas64(tags + victim, t[victim], ident | (unsigned long long)depth)) {
+                    } else if (cas64(tags + victim, pick_tag(t, victim), ident | (unsigned long long)depth)) {
                         // not ready: nobody else touches the entry until the
                         // key is complete and the ready bit is published
                         uint32_t *kw = P.dd_key + (bucket * kDedupBucket + victim) * 16;
-                        for (int w = 0; w < nw; ++w) st32(kw + w, child_spi_word(pw, w, m));
+#pragma unroll 1
+                        for (int w = 0; w < nw; ++w) st32(kw + w, L.ckey[w][lane]);
                         complete_before();
                         st64(tags + victim, ident | kTagReady | (unsigned long long)depth);
                         res = kDedupInserted;
@@ -579,7 +709,7 @@ __device__ int dedup_admit(const SearchParams &P, bool active, uint32_t cfg, con
 
 // Deduplicates and pushes the staged children (descending triple order).
 // root_stage: children of the start node (not counted as admitted).
-__device__ void flush_stage(const SearchParams &P, WaveState &W, WaveLds &L, int count, int n, int sw,
+__device__ __forceinline__ void flush_stage(const SearchParams &P, WaveState &W, WaveLds &L, int count, int n, int sw,
                             int T, bool root_stage) {
     const uint32_t cfg = L.node[0];
     const int d = (int)(L.node[1] & 0xFF);
@@ -597,7 +727,8 @@ __device__ void flush_stage(const SearchParams &P, WaveState &W, WaveLds &L, int
         m.na = spi[m.c]; m.nb = spi[m.a]; m.nc = spi[m.b];
         LaneTraffic tr{0, 0, 0, 0};
         PHASE_T0();
-        const int r = dedup_admit(P, act, cfg, L.node + kNodeHeaderWords, sw, m, d + 1, tr);
+        for (int q = 0; q < sw; ++q) L.ckey[q][W.lane] = child_spi_word(L.node + kNodeHeaderWords, q, m);
+        const int r = dedup_admit(P, L, W.lane, act, cfg, sw, d + 1, tr);
         PHASE_ADD(L, kPhaseDedup);
         lds_add(&L.traffic[2], (unsigned long long)wave_sum(tr.probes));
         lds_add(&L.traffic[3], (unsigned long long)wave_sum(tr.key_reads));
@@ -851,53 +982,72 @@ __global__ __launch_bounds__(256) void search_kernel(SearchParams P) {
         } else if (P.donate) {
             PHASE_T0();
             if (!idle) {
-                if (W.lane == 0) fetch_add(&P.ctrl[kCtlIdle].v, 1ULL);
+                // count first, then advertise: a claim always finds the wave
+                // already counted idle
+                if (W.lane == 0) {
+                    fetch_add(&P.ctrl[kCtlIdle].v, 1ULL);
+                    __hip_atomic_fetch_or(idle_word(P, W.wave >> 6), 1ULL << (W.wave & 63), __ATOMIC_RELAXED,
+                                          __HIP_MEMORY_SCOPE_AGENT);
+                }
                 idle = true;
-            }
-            // Thousands of idle waves poll the same control lines, so a poll
-            // is two loads by lane 0 with up to ~60 us of sleep between polls,
-            // and the termination test runs on every 4th empty poll only.
-            // Termination: the pool is empty at the first read (head == tail),
-            // every registered wave is idle at the second, and neither head nor
-            // tail moved by the third.  Work only leaves a wave through the
-            // pool (tail moves) and only enters an idle wave by a take (head
-            // moves, after the taker left the idle count), so no wave can hold
-            // a node once all three reads agree.
-            int work = 0, done = 0;
-            if (W.lane == 0) {
-                const unsigned long long h0 = ctl_load(P, kCtlPoolHead);
-                complete_after();
-                const unsigned long long t0 = ctl_load(P, kCtlPoolTail);
-                work = h0 < t0 ? 1 : 0;
-                if (!work && (++idle_polls & 3) == 0) {
-                    complete_after();
-                    const unsigned long long idl = ctl_load(P, kCtlIdle);
-                    complete_after();
-                    const unsigned long long reg = ctl_load(P, kCtlRegistered);
-                    complete_after();
-                    const unsigned long long h2 = ctl_load(P, kCtlPoolHead);
-                    complete_after();
-                    const unsigned long long t2 = ctl_load(P, kCtlPoolTail);
-                    done = (idl == reg && h2 == h0 && t2 == t0) ? 1 : 0;
-                }
-            }
-            idle_polls = __shfl(idle_polls, 0);
-            if (__shfl(work, 0)) {
-                if (W.lane == 0) fetch_add(&P.ctrl[kCtlIdle].v, ~0ULL);
-                idle = false;
                 backoff = 1;
-                if (take_one(P, W, L)) {
-                    have = true;
-                    PHASE_ADD(L, kPhaseShare);
-                } else {
-                    PHASE_ADD(L, kPhaseShare);
-                    continue;
+            }
+            if (receive_nodes(P, W)) {
+                // the claiming wave already took this wave out of the idle count
+                idle = false;
+                PHASE_ADD(L, kPhaseShare);
+                continue;
+            }
+            // every 4th empty poll: drain the spill pool, or end the launch.
+            // The launch is over when the pool is empty at the first read
+            // (head == tail), every registered wave is idle at the second,
+            // and neither head nor tail moved by the third: work leaves a
+            // busy wave only through an inbox (the claim takes the receiver
+            // out of the idle count first) or the pool (the tail moves), and
+            // enters an idle wave from the pool only after the taker left
+            // the idle count (then the head moves).
+            if ((++idle_polls & 3) == 0) {
+                int work = 0, done = 0;
+                if (W.lane == 0) {
+                    const unsigned long long h0 = ctl_load(P, kCtlPoolHead);
+                    complete_after();
+                    const unsigned long long t0 = ctl_load(P, kCtlPoolTail);
+                    if (h0 < t0) {
+                        work = 1;
+                    } else {
+                        complete_after();
+                        const unsigned long long reg = ctl_load(P, kCtlRegistered);
+                        complete_after();
+                        const unsigned long long idl = ctl_load(P, kCtlIdle);
+                        complete_after();
+                        const unsigned long long h2 = ctl_load(P, kCtlPoolHead);
+                        complete_after();
+                        const unsigned long long t2 = ctl_load(P, kCtlPoolTail);
+                        done = (idl == reg && h2 == h0 && t2 == t0) ? 1 : 0;
+                    }
+                    if (work) {
+                        // leave the idle set before taking: clearing the own
+                        // bit decides between this take and a claim already
+                        // made (whose nodes then arrive in the inbox)
+                        const unsigned long long bit = 1ULL << (W.wave & 63);
+                        work = (__hip_atomic_fetch_and(idle_word(P, W.wave >> 6), ~bit, __ATOMIC_RELAXED,
+                                                       __HIP_MEMORY_SCOPE_AGENT) & bit) ? 1 : 0;
+                        if (work) fetch_add(&P.ctrl[kCtlIdle].v, ~0ULL);
+                    }
                 }
-            } else {
                 if (__shfl(done, 0)) break;
-                if ((idle_polls & 3) == 0 && should_stop(P)) break;
+                if (__shfl(work, 0)) {
+                    idle = false;   // a failed take re-enters the idle set
+                    have = take_one(P, W, L);
+                    PHASE_ADD(L, kPhaseShare);
+                    if (!have) continue;
+                } else if (should_stop(P)) {
+                    break;
+                }
+            }
+            if (!have) {
                 for (int z = 0; z < backoff; ++z) __builtin_amdgcn_s_sleep(127);
-                if (backoff < 16) backoff *= 2;
+                if (backoff < (int)P.idle_backoff) backoff *= 2;
                 PHASE_ADD(L, kPhaseShare);
                 continue;
             }
@@ -928,22 +1078,13 @@ __global__ __launch_bounds__(256) void search_kernel(SearchParams P) {
         const int goal = expand(P, W, L, C, false);
         if (goal >= 0) report_found(P, W, L, cfg, d + 1, goal);
 
-        // donation: when other waves are idle, hand a few of the oldest nodes
-        // (the largest subtrees) to the shared pool; checked every 16
-        // expansions so the owner's depth-first order is mostly kept
-        if (P.donate && ((++iter & 15) == 0) && W.hi - W.lo >= 3) {
+        // donation: every share_mask + 1 expansions, a wave with at least two
+        // nodes hands up to half of them (at most share_max) to an idle wave
+        if (P.donate && ((++iter & P.share_mask) == 0) && W.hi - W.lo >= 2) {
             PHASE_T0();
-            unsigned long long idl = 0;
-            if (W.lane == 0) idl = ctl_load(P, kCtlIdle);
-            idl = (unsigned long long)__shfl((int)idl, 0);
-            if (idl > 0) {
-                uint32_t give = (W.hi - W.lo) / 2;
-                if (give > idl) give = (uint32_t)idl;
-                if (give > 4) give = 4;
-                for (uint32_t g = 0; g < give; ++g)
-                    if (!donate_one(P, W)) break;
-                if (W.lane == 0) fetch_add(&P.ctrl[kCtlDonations].v, (unsigned long long)give);
-            }
+            uint32_t give = (W.hi - W.lo) / 2;
+            if (give > P.share_max) give = P.share_max;
+            share_nodes(P, W, give, iter);
             PHASE_ADD(L, kPhaseShare);
         }
     }
@@ -973,7 +1114,9 @@ __global__ void reset_kernel(SearchParams P) {
         for (int m = 0; m < 16; ++m) r.moves[m] = 0;
         for (int c = 0; c < kNumCounters; ++c) r.counters[c] = 0;
     }
-    if (tid < (size_t)P.pool_cap) P.pool_seq[tid] = tid;
+    if (tid < (size_t)P.num_waves && P.inbox) P.inbox[tid * 16] = 0;
+    if (tid < (size_t)P.idle_words && P.idle_bits) P.idle_bits[tid * 16] = 0;
+    if (tid < (size_t)P.pool_cap && P.pool_seq) P.pool_seq[tid] = tid;
 }
 
 // Debug/validation kernel: odd-cycle count of every child of one node,
