@@ -7,10 +7,15 @@ table of 2^27 entries.  The configurations are uploaded once before the timed
 region and stay resident in HBM (TDP_FLAG_RESIDENT).
 
 Prints one JSON line: states expanded per second (value), cases per hour,
-the search kernel's HBM roofline figure (algorithmic bytes from the kernel's
-own traffic counters over its HIP-event duration) and the CPU baseline (the
-C restatement of the reference search, on a bounded sample of the same
+the search kernel's roofline figure (algorithmic queue + dedup bytes of the
+launch, from the kernel's own counters, over its HIP-event duration on the
+slot's stream), the measured HBM bytes per launch when the rocprofv3 counter
+summary of this workload is present under profiles/, and the CPU baseline
+(the C restatement of the reference search, on a bounded sample of the same
 workload, over the host's cores).
+
+``--copy-calibration MB`` instead runs a plain device copy of MB MiB (the
+known-size kernel that scales the FETCH_SIZE / WRITE_SIZE counters).
 """
 from __future__ import annotations
 
@@ -27,17 +32,43 @@ HBM_PEAK_GBS = 8000.0
 METRIC = "states expanded/s + cases/hour at depth 11, 1 MI355X; learned vs plain ordering"
 
 
-def algorithmic_bytes(stats: dict) -> int:
-    """Queue and dedup traffic one launch has to move:
-    node reads (pops + steals) and writes (pushes + donations, each donation
-    also reads the node); 8-byte dedup tags read (probes), stored keys
-    compared (key bytes plus the 8-byte tag re-read), and entries written
-    (key bytes, the tag CAS and the tag publication, 8 + 8 bytes)."""
+def algorithmic_bytes(stats: dict, counters: dict) -> int:
+    """Queue and dedup bytes of one launch, per expansion
+    B = S_node (pop) + a S_node (child writes) + (a + h) B_probe + c S_node
+    (a admitted, h dominated, c nodes moved between waves, each moved node
+    written and read once), where B_probe = 8 B of tag per child looked up
+    plus the stored key bytes compared on a tag match and the key + tag bytes
+    written on an insert.  Summed over the launch from the kernel's counters."""
     nb = int(stats["node_bytes"])
     kb = int(stats["key_bytes"])
-    node = nb * (int(stats["pops"]) + int(stats["pushes"]) + 2 * int(stats["donations"]) + int(stats["steals"]))
-    dedup = 8 * int(stats["probes"]) + (kb + 8) * int(stats["key_reads"]) + (kb + 16) * int(stats["key_writes"])
+    a = int(counters["admitted"].sum())
+    h = int(counters["dominated"].sum())
+    moved = int(stats["donations"]) + int(stats["spills"])
+    node = nb * (int(stats["pops"]) + int(stats["pushes"]) + 2 * moved)
+    dedup = 8 * (a + h) + kb * int(stats["key_reads"]) + (kb + 8) * int(stats["key_writes"])
     return node + dedup
+
+
+def measured_traffic(workload: str):
+    """HBM bytes per search launch from the committed rocprofv3 counter
+    summary of this workload (FETCH_SIZE + WRITE_SIZE, scaled by the copy
+    calibration), or None when it is absent."""
+    path = os.path.join(ROOT, "profiles", f"search_kernel_{workload}_hbm_bytes.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        return json.load(fh).get("hbm_bytes_per_launch")
+
+
+def copy_calibration(mb: int) -> None:
+    import torch
+    n = (mb << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda:0")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4, "dispatches": 3}))
 
 
 def cpu_baseline(wl, sample_cases: int, cap: int, threads: int):
@@ -55,7 +86,7 @@ def cpu_baseline(wl, sample_cases: int, cap: int, threads: int):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--cases", type=int, default=4096)
     ap.add_argument("--queue-mb", type=int, default=32768)
@@ -63,6 +94,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=48)
     ap.add_argument("--cpu-cap", type=int, default=20000)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -70,6 +102,10 @@ def main():
               "(configurations would shard across GPUs with no communication); use --gpus 1.",
               file=sys.stderr)
         sys.exit(2)
+
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
 
     import numpy as np
     import tdp1375_amd as T
@@ -97,9 +133,12 @@ def main():
               f"{T.last_launch_stats(0)['kernel_ms']:.1f} ms", file=sys.stderr, flush=True)
     verdicts = first.found.copy()
 
+    import torch  # device synchronisation around the timed region only
+
     exps = 0
     kernel_ms = []
     traffic = []
+    torch.cuda.synchronize()
     t0 = time.perf_counter()
     for s in range(args.steps):
         r = step(resident=True)
@@ -108,9 +147,10 @@ def main():
             raise RuntimeError("verdicts changed between steps")
         exps += int(r.counters["expansions"].sum())
         kernel_ms.append(st["kernel_ms"])
-        traffic.append(algorithmic_bytes(st))
+        traffic.append(algorithmic_bytes(st, r.counters))
         print(f"step {s}: {st['kernel_ms']:.1f} ms, {int(r.counters['expansions'].sum())} expansions",
               file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     T.destroy_slots()
 
@@ -129,15 +169,16 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "int32",
+        "dtype": "u8",
         "data": "synthetic",
         "config": {"workload": wl.name, "cases": len(cases), "n_range": [9, 36], "max_ratio": wl.max_ratio,
                    "max_depth": wl.max_depth, "dedup_table_size": 1 << args.dedup_log2,
                    "queue_mb": args.queue_mb, "ordering": "plain", "found": int(verdicts.sum())},
         "cases_per_hour": cases_per_hour,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "search_kernel", "avg_kernel_ms": 1e3 * avg_kernel_s},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(wl.name),
+                     "kernel": "search_kernel", "avg_kernel_ms": 1e3 * avg_kernel_s,
+                     "algorithmic_bytes_per_launch": sum(traffic) / len(traffic)},
     }
     if not args.no_cpu_baseline:
         threads = max(1, min(16, os.cpu_count() or 1))
