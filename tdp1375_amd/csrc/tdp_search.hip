@@ -49,6 +49,9 @@ enum DedupOutcome { kDedupPending = -1, kDedupDominated = 0, kDedupImproved = 1,
 struct WaveLds {
     uint32_t node[48];
     uint32_t posinfo[64];
+    uint8_t memb[64];                        // cycle members in spi order, cycles of length >= 3
+    uint8_t posof[64];                       // position of each symbol in pi
+    uint32_t cyc[24];                        // per cycle: first candidate | len << 16 | member offset << 24
     uint32_t stage[kStage];
     uint32_t ckey[16][kWave];                // child spi words of each lane's staged child
     unsigned long long cnt[kNumCounters];    // counters of the current configuration
@@ -244,7 +247,7 @@ __device__ __forceinline__ int transposed_src_index(int p, int i, int j, int k) 
 
 // Lane s handles symbol s: cycle id (min symbol) and distance to it by
 // pointer jumping over ds_bpermute, then posinfo[p] = info(pi[p]).
-__device__ void build_posinfo(WaveLds &L, int n, int sw, int lane) {
+__device__ uint32_t build_posinfo(WaveLds &L, int n, int sw, int lane) {
     const uint8_t *spi = reinterpret_cast<const uint8_t *>(L.node + kNodeHeaderWords);
     const uint8_t *pi = spi + 4 * sw;
     const int s = lane;
@@ -276,6 +279,7 @@ __device__ void build_posinfo(WaveLds &L, int n, int sw, int lane) {
     const int x = (s < n) ? (int)pi[s] : s;
     L.posinfo[lane] = (uint32_t)__shfl((int)info, x);
     wave_sync();
+    return info;   // of symbol `lane`
 }
 
 struct WaveState {
@@ -786,6 +790,104 @@ __device__ void report_found(const SearchParams &P, WaveState &W, WaveLds &L, in
     }
 }
 
+// Children of an expansion that needs a gain of +2 on a node whose cycles
+// all have odd length.  Such a child must take its three symbols from one
+// cycle: three different cycles merge into one (delta -2 for odd lengths) and
+// two symbols of one odd cycle with one of another odd cycle leave two cycles
+// of lengths of equal parity (delta <= 0).  So instead of all C(n,3) triples
+// only the sum of C(L,3) over the cycles is evaluated, with the same parity
+// delta as the full scan.  The result is the one of the full scan: the goal is
+// the largest goal triple index, and the admitted children are staged in
+// descending triple order.  Returns false (nothing done) when the node has an
+// even cycle or the candidates are too many for the stage; otherwise sets
+// scount / goal_t as the scan would.
+constexpr int kCycleLocalMax = 512;
+
+__device__ bool cycle_local_children(WaveState &W, WaveLds &L, uint32_t sinfo, int n, int sw, int T, int e,
+                                     uint32_t gmask, uint32_t amask, bool root_mode, int &scount, int &goal_t) {
+    const int s = W.lane;
+    const int root = (int)(sinfo & 63u), pos = (int)((sinfo >> 8) & 63u), len = (int)((sinfo >> 16) & 127u);
+    const bool is_root = s < n && root == s;
+    if (__ballot(is_root && (len & 1) == 0) != 0) return false;
+    const bool big = is_root && len >= 3;
+    int kc = big ? choose3(len) : 0;
+    int lc = big ? len : 0;
+    // inclusive prefix sums over the lanes (cycles in root order)
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+        const int kv = __shfl_up(kc, off), lv = __shfl_up(lc, off);
+        if (s >= off) { kc += kv; lc += lv; }
+    }
+    const int K = __shfl(kc, kWave - 1);
+    if (K > kCycleLocalMax || 4 * K > T) return false;
+    const unsigned long long roots = __ballot(big);
+    const int kfirst = kc - (big ? choose3(len) : 0);
+    const int moff = lc - (big ? len : 0);
+    if (big) {
+        const int c = __popcll(roots & ((1ULL << s) - 1ULL));
+        L.cyc[c] = (uint32_t)kfirst | ((uint32_t)len << 16) | ((uint32_t)moff << 24);
+    }
+    const int my_off = __shfl(moff, root);
+    if (s < n && len >= 3) L.memb[my_off + pos] = (uint8_t)s;
+    const uint8_t *pi = reinterpret_cast<const uint8_t *>(L.node + kNodeHeaderWords) + 4 * sw;
+    if (s < n) L.posof[pi[s]] = (uint8_t)s;
+    wave_sync();
+    const int ncyc = __popcll(roots);
+    uint32_t *tmp_t = &L.ckey[0][0];
+    uint32_t *tmp_e = &L.ckey[8][0];
+    int best = -1, cnt = 0;
+    for (int g0 = 0; g0 < K; g0 += kWave) {
+        const int g = g0 + s;
+        const bool valid = g < K;
+        int c = 0;
+        for (int q = 1; q < ncyc; ++q) c = ((int)(L.cyc[q] & 0xFFFFu) <= g) ? q : c;
+        const uint32_t cw = L.cyc[c];
+        const int cl = (int)((cw >> 16) & 0xFFu), co = (int)(cw >> 24);
+        int r = valid ? g - (int)(cw & 0xFFFFu) : 0;
+        int p2 = 2;
+        while (p2 + 1 < cl && choose3(p2 + 1) <= r) ++p2;
+        r -= choose3(p2);
+        int p1 = 1;
+        while (p1 + 1 < p2 && choose2(p1 + 1) <= r) ++p1;
+        const int p0 = r - choose2(p1);
+        int a = L.posof[L.memb[co + p0]], b = L.posof[L.memb[co + p1]], x = L.posof[L.memb[co + p2]];
+        const int i = min(a, min(b, x)), k = max(a, max(b, x));
+        const int j = a + b + x - i - k;
+        const int delta = parity_delta(L.posinfo[i], L.posinfo[j], L.posinfo[k]);
+        const uint32_t di = (uint32_t)(delta + 2) >> 1;
+        const bool goal = valid && !root_mode && ((gmask >> di) & 1u);
+        const bool adm = valid && ((amask >> di) & 1u);
+        const int t = lex_index(n, T, i, j, k);
+        if (goal) best = max(best, t);
+        const unsigned long long am = __ballot(adm);
+        const unsigned long long below = (s == 0) ? 0ULL : (am & (~0ULL >> (64 - s)));
+        if (adm) {
+            tmp_t[cnt + __popcll(below)] = (uint32_t)t;
+            tmp_e[cnt + __popcll(below)] = stage_entry(i, j, k, e + delta);
+        }
+        cnt += __popcll(am);
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) best = max(best, __shfl_xor(best, off));
+    if (best >= 0) {
+        goal_t = best;
+        return true;
+    }
+    wave_sync();
+    // descending triple order: the rank of an entry is the number of larger
+    // triple indices (all distinct)
+    for (int idx = s; idx < cnt; idx += kWave) {
+        const uint32_t tt = tmp_t[idx];
+        int rank = 0;
+        for (int q = 0; q < cnt; ++q) rank += tmp_t[q] > tt ? 1 : 0;
+        L.stage[rank] = tmp_e[idx];
+    }
+    if (root_mode) lds_add(&L.cnt[kCntRootsLive], (unsigned long long)cnt);
+    wave_sync();
+    scount = cnt;
+    return true;
+}
+
 // Evaluates every child of L.node.  Normal mode: returns the goal triple
 // (terminal expansion) or -1 after staging/pushing admissible children.
 // Root mode: stages and pushes every live root.
@@ -797,9 +899,10 @@ __device__ int expand(const SearchParams &P, WaveState &W, WaveLds &L, const Con
     const int D = (int)C.max_depth;
     const int d = (int)(L.node[1] & 0xFF);
     const int e = (int)((L.node[1] >> 8) & 0xFF);
+    uint32_t sinfo;
     {
         PHASE_T0();
-        build_posinfo(L, n, sw, W.lane);
+        sinfo = build_posinfo(L, n, sw, W.lane);
         PHASE_ADD(L, kPhaseAnalyse);
     }
     // Both predicates are monotone in the odd-cycle count and a move changes it
@@ -868,7 +971,8 @@ __device__ int expand(const SearchParams &P, WaveState &W, WaveLds &L, const Con
         }
         return false;
     };
-    for_each_child_chunk(W.colex, n, T, W.lane, visit);
+    if (!(plus2_only && cycle_local_children(W, L, sinfo, n, sw, T, e, gmask, amask, root_mode, scount, goal_t)))
+        for_each_child_chunk(W.colex, n, T, W.lane, visit);
     if (W.stop) return -1;
     PHASE_ADD(L, kPhaseScan);
     if (goal_t >= 0) return goal_t;

@@ -89,3 +89,40 @@ def test_positions_to_symbols_replay():
     # after the first block swap pi = [2, 3, 0, 1, 4, 5]
     assert moves[1] == (3, 1, 5)
     assert pi == [0, 1, 2, 3, 4, 5]
+
+
+def test_plus2_children_of_odd_cycle_nodes_keep_one_cycle():
+    """Premise of the kernel's cycle-local enumeration: on a node whose cycles
+    all have odd length, every child with two more odd cycles moves three
+    symbols of one cycle (checked with the oracle's O(n) recount of every
+    child)."""
+    rng = np.random.default_rng(5)
+    for it in range(60):
+        n = int(rng.integers(5, 40))
+        lengths, left = [], n
+        while left > 0:
+            L = int(rng.choice([1, 3, 5, 7]))
+            L = min(L, left if left % 2 else left - 1) if left > 1 else 1
+            lengths.append(L)
+            left -= L
+        sym = [int(x) for x in rng.permutation(n)]
+        cycles, p = [], 0
+        for L in lengths:
+            cycles.append(sym[p:p + L])
+            p += L
+        spi = W.one_line(cycles, n)
+        pi = [int(x) for x in rng.permutation(n)]
+        e = W.odd_cycle_count(spi)
+        assert e == len(cycles)
+        cyc_of = {s: c for c, cyc in enumerate(cycles) for s in cyc}
+        kids = oracle.children_even(pi, spi)
+        t = 0
+        plus2 = 0
+        for i in range(n):
+            for j in range(i + 1, n):
+                for k in range(j + 1, n):
+                    if kids[t] == e + 2:
+                        plus2 += 1
+                        assert cyc_of[pi[i]] == cyc_of[pi[j]] == cyc_of[pi[k]], (it, i, j, k)
+                    t += 1
+        assert plus2 <= sum(L * (L - 1) * (L - 2) // 6 for L in lengths)
