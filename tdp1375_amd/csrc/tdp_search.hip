@@ -195,34 +195,56 @@ __device__ __forceinline__ int child_delta(uint32_t ia, uint32_t ib, uint32_t ic
 
 // Enumerates the children in descending lexicographic order, one chunk of
 // 64 consecutive offsets per call of visit(valid, i, j, k, obase) (lane l
-// holds offset obase + l); visit returns true to stop.  Blocks m <= 11
-// (first index i = n-1-m) hold fewer than 64 pairs each, so the first
-// C(12,3) = 220 offsets run with a per-lane block cursor; larger blocks run
-// one wave-uniform block at a time with lanes over its colex pairs.
+// holds offset obase + l).  Blocks m <= 11 (first index i = n-1-m) hold fewer
+// than 64 pairs each, so the first C(12,3) = 220 offsets run with a per-lane
+// block cursor; larger blocks run one wave-uniform block at a time with lanes
+// over its colex pairs.  visit returns kScanGo to continue, kScanStop to end
+// the scan, kScanPause to end it with (mb, qb) set to the next chunk so that
+// a later call resumes there (mb < 0: inside the first 220 offsets, qb the
+// next offset).  Pausing lets the caller flush the stage outside the scan, so
+// the scan's and the flush's registers are never live together.
+enum { kScanGo = 0, kScanStop = 1, kScanPause = 2 };
+
 template <typename Visit>
-__device__ __forceinline__ void for_each_child_chunk(const uint16_t *colex, int n, int T, int lane, Visit visit) {
-    const int prefix = T < 220 ? T : 220;
-    bool done = false;
-    int m = 2;
-    for (int obase = 0; obase < prefix && !done; obase += kWave) {
-        const int o = obase + lane;
-        const bool valid = o < prefix;
-        int i, j, k;
-        triple_at(colex, n, m, o, valid, i, j, k);
-        done = visit(valid, i, j, k, obase);
+__device__ __forceinline__ int scan_children(const uint16_t *colex, int n, int T, int lane, int &mb, int &qb,
+                                             Visit visit) {
+    if (mb < 0) {
+        const int prefix = T < 220 ? T : 220;
+        int m = 2;
+        for (int obase = qb; obase < prefix; obase += kWave) {
+            const int o = obase + lane;
+            const bool valid = o < prefix;
+            int i, j, k;
+            triple_at(colex, n, m, o, valid, i, j, k);
+            const int r = visit(valid, i, j, k, obase);
+            if (r == kScanStop) return r;
+            if (r == kScanPause) {
+                qb = obase + kWave;
+                return r;
+            }
+        }
+        mb = 12;
+        qb = 0;
     }
-    for (int mb = 12; mb < n && !done; ++mb) {
+    for (; mb < n; ++mb, qb = 0) {
         const int pairs = choose2(mb), o0 = choose3(mb), i = n - 1 - mb;
-        for (int qb = 0; qb < pairs && !done; qb += kWave) {
+        for (; qb < pairs; qb += kWave) {
             const int q = qb + lane;
             const bool valid = q < pairs;
             const uint32_t pair = colex[valid ? q : 0];
-            done = visit(valid, i, n - 1 - (int)(pair >> 8), n - 1 - (int)(pair & 0xFF), o0 + qb);
+            const int r = visit(valid, i, n - 1 - (int)(pair >> 8), n - 1 - (int)(pair & 0xFF), o0 + qb);
+            if (r == kScanStop) return r;
+            if (r == kScanPause) {
+                qb += kWave;
+                if (qb >= pairs) { ++mb; qb = 0; }
+                return r;
+            }
         }
     }
+    return kScanGo;
 }
 
-__device__ void fill_colex(uint16_t *colex) {
+__device__ __forceinline__ void fill_colex(uint16_t *colex) {
     // q -> (u, v) with v the largest such that C(v,2) <= q
     for (int q = threadIdx.x; q < kColexPairs; q += blockDim.x) {
         int v = 1;
@@ -247,7 +269,7 @@ __device__ __forceinline__ int transposed_src_index(int p, int i, int j, int k) 
 
 // Lane s handles symbol s: cycle id (min symbol) and distance to it by
 // pointer jumping over ds_bpermute, then posinfo[p] = info(pi[p]).
-__device__ uint32_t build_posinfo(WaveLds &L, int n, int sw, int lane) {
+__device__ __forceinline__ uint32_t build_posinfo(WaveLds &L, int n, int sw, int lane) {
     const uint8_t *spi = reinterpret_cast<const uint8_t *>(L.node + kNodeHeaderWords);
     const uint8_t *pi = spi + 4 * sw;
     const int s = lane;
@@ -296,12 +318,12 @@ __device__ __forceinline__ bool past_deadline(const WaveState &W) {
     return W.deadline != 0 && __builtin_amdgcn_s_memrealtime() > W.deadline;
 }
 
-__device__ void set_watchdog(const SearchParams &P, WaveState &W) {
+__device__ __forceinline__ void set_watchdog(const SearchParams &P, WaveState &W) {
     if (W.lane == 0) __hip_atomic_store(&P.ctrl[kCtlStop].v, 2ULL, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     W.stop = true;
 }
 
-__device__ void flush_counters(const SearchParams &P, WaveState &W, WaveLds &L) {
+__device__ __forceinline__ void flush_counters(const SearchParams &P, WaveState &W, WaveLds &L) {
     wave_sync();
     if (W.cur_cfg >= 0 && W.lane < kNumCounters) {
         const unsigned long long v = L.cnt[W.lane];
@@ -315,7 +337,7 @@ __device__ __forceinline__ uint32_t *stack_slot(const SearchParams &P, const Wav
     return P.stacks + ((size_t)W.wave * P.stack_cap + (size_t)(idx & (P.stack_cap - 1))) * P.node_words;
 }
 
-__device__ void set_overflow(const SearchParams &P, WaveState &W) {
+__device__ __forceinline__ void set_overflow(const SearchParams &P, WaveState &W) {
     if (W.lane == 0) __hip_atomic_store(&P.ctrl[kCtlStop].v, 1ULL, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     W.stop = true;
 }
@@ -338,7 +360,7 @@ __device__ __forceinline__ unsigned long long *idle_word(const SearchParams &P, 
 }
 
 // Claims one idle wave; returns its index or -1 (wave-uniform).
-__device__ int claim_idle_wave(const SearchParams &P, const WaveState &W, unsigned salt) {
+__device__ __forceinline__ int claim_idle_wave(const SearchParams &P, const WaveState &W, unsigned salt) {
     int r = -1;
     if (W.lane == 0) {
         const int words = (int)P.idle_words;
@@ -363,7 +385,7 @@ __device__ int claim_idle_wave(const SearchParams &P, const WaveState &W, unsign
 
 // Hands up to `most` of the oldest nodes of this wave's stack to an idle wave.
 // Returns the number handed over (0: no idle wave).
-__device__ uint32_t share_nodes(const SearchParams &P, WaveState &W, uint32_t most, unsigned salt) {
+__device__ __forceinline__ uint32_t share_nodes(const SearchParams &P, WaveState &W, uint32_t most, unsigned salt) {
     if (most == 0) return 0;
     const int r = claim_idle_wave(P, W, salt);
     if (r < 0) return 0;
@@ -384,7 +406,7 @@ __device__ uint32_t share_nodes(const SearchParams &P, WaveState &W, uint32_t mo
 
 // Moves the nodes published in this wave's inbox onto its stack (which is
 // empty: the wave was idle).  Returns false when the inbox is empty.
-__device__ bool receive_nodes(const SearchParams &P, WaveState &W) {
+__device__ __forceinline__ bool receive_nodes(const SearchParams &P, WaveState &W) {
     unsigned long long cnt = 0;
     if (W.lane == 0) cnt = ld_relaxed(inbox_line(P, W.wave));
     const int k = uniform(__shfl((int)cnt, 0));
@@ -410,7 +432,7 @@ __device__ bool receive_nodes(const SearchParams &P, WaveState &W) {
 
 // Moves the oldest node of the private stack into the spill pool.  Returns
 // false when the pool is full.
-__device__ bool donate_one(const SearchParams &P, WaveState &W) {
+__device__ __forceinline__ bool donate_one(const SearchParams &P, WaveState &W) {
     if (!P.pool_seq) return false;
     unsigned long long ticket = 0;
     int ok = 0;
@@ -457,7 +479,7 @@ __device__ bool donate_one(const SearchParams &P, WaveState &W) {
 }
 
 // Takes one node from the shared pool into L.node.
-__device__ bool take_one(const SearchParams &P, WaveState &W, WaveLds &L) {
+__device__ __forceinline__ bool take_one(const SearchParams &P, WaveState &W, WaveLds &L) {
     unsigned long long ticket = 0;
     int ok = 0;
     if (W.lane == 0) {
@@ -497,7 +519,7 @@ __device__ bool take_one(const SearchParams &P, WaveState &W, WaveLds &L) {
 // Ensures room for one push on the private stack: a full stack hands its
 // oldest nodes to an idle wave, else spills them to the pool, else the launch
 // stops with a queue overflow.
-__device__ bool make_room(const SearchParams &P, WaveState &W) {
+__device__ __forceinline__ bool make_room(const SearchParams &P, WaveState &W) {
     if (W.hi - W.lo < P.stack_cap) return true;
     unsigned long long reason = 1;   // 1: no sharing, 2: pool full
     if (P.donate) {
@@ -522,7 +544,7 @@ __device__ bool make_room(const SearchParams &P, WaveState &W) {
 }
 
 // Pushes the child (t, i, j, k, even) of the node in L.node.
-__device__ void push_child(const SearchParams &P, WaveState &W, WaveLds &L, int n, int sw,
+__device__ __forceinline__ void push_child(const SearchParams &P, WaveState &W, WaveLds &L, int n, int sw,
                            int t, int i, int j, int k, int even) {
     if (!make_room(P, W)) return;
     const uint32_t *pw = L.node;
@@ -594,6 +616,13 @@ __device__ __forceinline__ unsigned long long pick_tag(const unsigned long long 
 
 // The child's key words are in L.ckey[0..nw)[lane] (written by the caller),
 // so they do not occupy registers across the probe loop.
+// Key words 2w and 2w+1 of a lane's child as one 64-bit word (0 past nw).
+__device__ __forceinline__ unsigned long long key_pair(const WaveLds &L, int lane, int w, int nw) {
+    const unsigned long long lo = L.ckey[2 * w][lane];
+    const unsigned long long hi = (2 * w + 1 < nw) ? L.ckey[2 * w + 1][lane] : 0u;
+    return lo | (hi << 32);
+}
+
 __device__ __forceinline__ int dedup_admit(const SearchParams &P, WaveLds &L, int lane, bool active, uint32_t cfg,
                                            int nw, int depth, LaneTraffic &tr) {
     if (P.dd_buckets == 0) return active ? kDedupUninserted : kDedupDominated;
@@ -632,10 +661,17 @@ __device__ __forceinline__ int dedup_admit(const SearchParams &P, WaveLds &L, in
                 const int q = __ffs(cand) - 1;
                 cand &= cand - 1;
                 const unsigned long long tq = pick_tag(t, q);
-                const uint32_t *kw = keys + q * 16;
+                // the key as 64-bit words, all loads in flight at once (an odd
+                // word count is padded with a zero word on both sides)
+                const unsigned long long *kw = reinterpret_cast<const unsigned long long *>(keys + q * 16);
+                const int nw2 = (nw + 1) >> 1;
+                unsigned long long kv[8];
+#pragma unroll
+                for (int w = 0; w < 8; ++w) kv[w] = w < nw2 ? ld_relaxed(kw + w) : 0ULL;
                 bool eq = true;
-#pragma unroll 1
-                for (int w = 0; w < nw; ++w) eq &= ld32(kw + w) == L.ckey[w][lane];
+#pragma unroll
+                for (int w = 0; w < 8; ++w)
+                    if (w < nw2) eq &= kv[w] == key_pair(L, lane, w, nw);
                 tr.key_reads++;
                 if (!eq) continue;
                 // the key read belongs to this entry only if the tag did not
@@ -692,9 +728,9 @@ __device__ __forceinline__ int dedup_admit(const SearchParams &P, WaveLds &L, in
                     } else if (cas64(tags + victim, pick_tag(t, victim), ident | (unsigned long long)depth)) {
                         // not ready: nobody else touches the entry until the
                         // key is complete and the ready bit is published
-                        uint32_t *kw = P.dd_key + (bucket * kDedupBucket + victim) * 16;
-#pragma unroll 1
-                        for (int w = 0; w < nw; ++w) st32(kw + w, L.ckey[w][lane]);
+                        unsigned long long *kw =
+                            reinterpret_cast<unsigned long long *>(P.dd_key + (bucket * kDedupBucket + victim) * 16);
+                        for (int w = 0; w < ((nw + 1) >> 1); ++w) st64(kw + w, key_pair(L, lane, w, nw));
                         complete_before();
                         st64(tags + victim, ident | kTagReady | (unsigned long long)depth);
                         res = kDedupInserted;
@@ -766,7 +802,7 @@ __device__ __forceinline__ void flush_stage(const SearchParams &P, WaveState &W,
     }
 }
 
-__device__ void report_found(const SearchParams &P, WaveState &W, WaveLds &L, int cfg, int depth,
+__device__ __forceinline__ void report_found(const SearchParams &P, WaveState &W, WaveLds &L, int cfg, int depth,
                              int extra_t) {
     int won = 0;
     if (W.lane == 0) {
@@ -803,7 +839,7 @@ __device__ void report_found(const SearchParams &P, WaveState &W, WaveLds &L, in
 // scount / goal_t as the scan would.
 constexpr int kCycleLocalMax = 512;
 
-__device__ bool cycle_local_children(WaveState &W, WaveLds &L, uint32_t sinfo, int n, int sw, int T, int e,
+__device__ __forceinline__ bool cycle_local_children(WaveState &W, WaveLds &L, uint32_t sinfo, int n, int sw, int T, int e,
                                      uint32_t gmask, uint32_t amask, bool root_mode, int &scount, int &goal_t) {
     const int s = W.lane;
     const int root = (int)(sinfo & 63u), pos = (int)((sinfo >> 8) & 63u), len = (int)((sinfo >> 16) & 127u);
@@ -891,7 +927,7 @@ __device__ bool cycle_local_children(WaveState &W, WaveLds &L, uint32_t sinfo, i
 // Evaluates every child of L.node.  Normal mode: returns the goal triple
 // (terminal expansion) or -1 after staging/pushing admissible children.
 // Root mode: stages and pushes every live root.
-__device__ int expand(const SearchParams &P, WaveState &W, WaveLds &L, const ConfigDesc &C,
+__device__ __forceinline__ int expand(const SearchParams &P, WaveState &W, WaveLds &L, const ConfigDesc &C,
                       bool root_mode) {
     const int n = (int)C.n;
     const int sw = (int)P.spi_words;
@@ -936,12 +972,12 @@ __device__ int expand(const SearchParams &P, WaveState &W, WaveLds &L, const Con
     const bool plus2_only = (gmask | amask) == 4u;
     // Visits the chunk whose lane l holds the child at descending offset
     // obase + l (lexicographic index T-1-obase-l); true ends the scan.
-    auto visit = [&](bool valid, int i, int j, int k, int obase) -> bool {
+    auto visit = [&](bool valid, int i, int j, int k, int obase) -> int {
         const uint32_t ia = L.posinfo[i], ib = L.posinfo[j], ic = L.posinfo[k];
         int delta;
         if (plus2_only) {
             const bool shared = hits_a_cycle_twice(ia, ib, ic);
-            if (__ballot(valid && shared) == 0) return false;
+            if (__ballot(valid && shared) == 0) return kScanGo;
             delta = shared ? parity_delta(ia, ib, ic) : -2;
         } else {
             delta = child_delta(ia, ib, ic);
@@ -953,7 +989,7 @@ __device__ int expand(const SearchParams &P, WaveState &W, WaveLds &L, const Con
             const unsigned long long gm = __ballot(goal);
             if (gm) {
                 goal_t = T - 1 - (obase + __ffsll((long long)gm) - 1);
-                return true;
+                return kScanStop;
             }
         }
         const unsigned long long am = __ballot(adm);
@@ -962,17 +998,20 @@ __device__ int expand(const SearchParams &P, WaveState &W, WaveLds &L, const Con
             if (adm) L.stage[scount + __popcll(below)] = stage_entry(i, j, k, e + delta);
             scount += __popcll(am);
             if (root_mode) lds_add(&L.cnt[kCntRootsLive], __popcll(am));
-            if (scount > kStage - kWave) {
-                wave_sync();
-                flush_stage(P, W, L, scount, n, sw, T, root_mode);
-                scount = 0;
-                if (W.stop) return true;
-            }
+            if (scount > kStage - kWave) return kScanPause;
         }
-        return false;
+        return kScanGo;
     };
-    if (!(plus2_only && cycle_local_children(W, L, sinfo, n, sw, T, e, gmask, amask, root_mode, scount, goal_t)))
-        for_each_child_chunk(W.colex, n, T, W.lane, visit);
+    if (!(plus2_only && cycle_local_children(W, L, sinfo, n, sw, T, e, gmask, amask, root_mode, scount, goal_t))) {
+        int mb = -1, qb = 0;
+        while (scan_children(W.colex, n, T, W.lane, mb, qb, visit) == kScanPause) {
+            // stage full: flush, then resume the scan at the next chunk
+            wave_sync();
+            flush_stage(P, W, L, scount, n, sw, T, root_mode);
+            scount = 0;
+            if (W.stop) return -1;
+        }
+    }
     if (W.stop) return -1;
     PHASE_ADD(L, kPhaseScan);
     if (goal_t >= 0) return goal_t;
@@ -988,18 +1027,18 @@ __device__ int expand(const SearchParams &P, WaveState &W, WaveLds &L, const Con
     return -1;
 }
 
-__device__ void switch_config(const SearchParams &P, WaveState &W, WaveLds &L, int cfg) {
+__device__ __forceinline__ void switch_config(const SearchParams &P, WaveState &W, WaveLds &L, int cfg) {
     if (cfg != W.cur_cfg) {
         flush_counters(P, W, L);
         W.cur_cfg = cfg;
     }
 }
 
-__device__ bool config_found(const SearchParams &P, int cfg) {
+__device__ __forceinline__ bool config_found(const SearchParams &P, int cfg) {
     return __hip_atomic_load(&P.results[cfg].found, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
 }
 
-__device__ bool should_stop(const SearchParams &P) {
+__device__ __forceinline__ bool should_stop(const SearchParams &P) {
     int s = 0;
     if ((threadIdx.x & 63) == 0) {
         s = ctl_load(P, kCtlStop) != 0;
@@ -1012,7 +1051,11 @@ __device__ bool should_stop(const SearchParams &P) {
 
 }  // namespace
 
-__global__ __launch_bounds__(256) void search_kernel(SearchParams P) {
+// Everything is inlined into the kernel: a call would pass the by-value
+// parameter block through scratch memory and LDS references as 64-bit flat
+// pointers.  Three waves per SIMD (<= 168 VGPRs) costs a few spilled
+// registers; four would spill heavily.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void search_kernel(SearchParams P) {
     __shared__ WaveLds lds[kWavesPerBlock];
     __shared__ uint16_t colex[kColexPairs];
     fill_colex(colex);
@@ -1249,10 +1292,11 @@ __global__ __launch_bounds__(64) void eval_children_kernel(const uint8_t *pi, co
     wave_sync();
     build_posinfo(L, n, sw, lane);
     // same enumeration and delta as the search scan, written at t = T-1-o
-    for_each_child_chunk(colex, n, T, lane, [&](bool valid, int i, int j, int k, int obase) {
+    int mb = -1, qb = 0;
+    scan_children(colex, n, T, lane, mb, qb, [&](bool valid, int i, int j, int k, int obase) -> int {
         const int delta = child_delta(L.posinfo[i], L.posinfo[j], L.posinfo[k]);
         if (valid) out[T - 1 - (obase + lane)] = e + delta;
-        return false;
+        return kScanGo;
     });
 }
 
