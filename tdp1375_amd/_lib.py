@@ -119,7 +119,7 @@ def load() -> ctypes.CDLL:
         L.tdp_host_predicate_tables.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_int,
                                                 ctypes.c_int, ctypes.POINTER(ctypes.c_uint64),
                                                 ctypes.POINTER(ctypes.c_uint64)]
-        L.tdp_set_tuning.argtypes = [ctypes.c_int] * 4
+        L.tdp_set_tuning.argtypes = [ctypes.c_int] * 5
         L.tdp_device_count.argtypes = []
         _lib = L
         return L

@@ -287,6 +287,7 @@ struct Tuning {
     int share_interval = 16;  // expansions between donation checks (power of two)
     int share_max = 4;        // nodes per donation
     int idle_backoff = 16;    // longest idle sleep, x 127 s_sleep units
+    int max_active = 512;     // configurations per persistent launch
 };
 std::mutex g_tuning_mutex;
 Tuning g_tuning;
@@ -549,15 +550,6 @@ void run_launch(Slot &s, const std::vector<CaseInput> &cases, uint32_t flags, bo
         s.share_lines_cap = lines_needed;
     }
 
-    // dedup epoch (replaces the per-search table memset, sorting_search.cu:744-747)
-    if (s.dedup_size > 0) {
-        s.epoch++;
-        if (s.epoch > 0xFFFF) {
-            HIP_CHECK(hipMemsetAsync(s.dd_tag.get(), 0, (size_t)s.dd_buckets * kDedupBucket * 8, s.stream));
-            s.epoch = 1;
-        }
-    }
-
     SearchParams P{};
     P.cfgs = as<ConfigDesc>(s.cfgs);
     P.results = as<ConfigResult>(s.results);
@@ -592,62 +584,93 @@ void run_launch(Slot &s, const std::vector<CaseInput> &cases, uint32_t flags, bo
     if (!resident)
         HIP_CHECK(hipMemcpyAsync(s.cfgs.get(), s.h_cfgs.get(), (size_t)count * sizeof(ConfigDesc),
                                  hipMemcpyHostToDevice, s.stream));
-    // one thread per configuration, wave (inbox line), pool slot and control line
-    const size_t reset_n = std::max<size_t>({(size_t)count, (size_t)waves, (size_t)pool_cap, (size_t)kNumControl});
-    hipLaunchKernelGGL(reset_kernel, dim3((unsigned)((reset_n + 255) / 256)), dim3(256), 0, s.stream, P);
-    HIP_CHECK(hipGetLastError());
-    HIP_CHECK(hipEventRecord(s.ev0, s.stream));
+
+    // Admission control (work-sharing mode): at most max_active
+    // configurations per persistent launch, launched back to back.  All
+    // configurations of a launch share the dedup table, and the entries of a
+    // finished round are freed at once by the next epoch, so bounding the
+    // configurations in flight keeps the table from thrashing.
+    const int round = (donate && tune.max_active > 0) ? std::min(count, tune.max_active) : count;
+    tdp_launch_stats acc{};
+    acc.waves = waves;
+    acc.node_bytes = node_bytes;
+    acc.key_bytes = (uint64_t)spi_words_for(max_n) * 4;
+    int stop_flag = 0;
+    bool cancelled = false;
     const int blocks = (waves + 3) / 4;
-    hipLaunchKernelGGL(search_kernel, dim3(blocks), dim3(256), 0, s.stream, P);
-    HIP_CHECK(hipGetLastError());
-    HIP_CHECK(hipEventRecord(s.ev1, s.stream));
+    for (int base = 0; base < count; base += round) {
+        const int sub = std::min(round, count - base);
+        // dedup epoch (replaces the per-search table memset, sorting_search.cu:744-747)
+        if (s.dedup_size > 0) {
+            s.epoch++;
+            if (s.epoch > 0xFFFF) {
+                HIP_CHECK(hipMemsetAsync(s.dd_tag.get(), 0, (size_t)s.dd_buckets * kDedupBucket * 8, s.stream));
+                s.epoch = 1;
+            }
+        }
+        P.cfgs = as<ConfigDesc>(s.cfgs) + base;
+        P.results = as<ConfigResult>(s.results) + base;
+        P.num_configs = sub;
+        P.epoch = s.epoch ? s.epoch : 1;
+        // one thread per configuration, wave (inbox line), pool slot and control line
+        const size_t reset_n = std::max<size_t>({(size_t)sub, (size_t)waves, (size_t)pool_cap, (size_t)kNumControl});
+        hipLaunchKernelGGL(reset_kernel, dim3((unsigned)((reset_n + 255) / 256)), dim3(256), 0, s.stream, P);
+        HIP_CHECK(hipGetLastError());
+        HIP_CHECK(hipEventRecord(s.ev0, s.stream));
+        hipLaunchKernelGGL(search_kernel, dim3(blocks), dim3(256), 0, s.stream, P);
+        HIP_CHECK(hipGetLastError());
+        HIP_CHECK(hipEventRecord(s.ev1, s.stream));
+        HIP_CHECK(hipMemcpyAsync(s.h_ctrl.get(), s.ctrl.get(), sizeof(ControlLine) * kNumControl,
+                                 hipMemcpyDeviceToHost, s.stream));
+
+        // wait: short spin, then poll with a small sleep; stop polling once a
+        // cancel was requested (the kernel sees the device flag and drains)
+        const auto t0 = std::chrono::steady_clock::now();
+        while (!cancel_requested(s)) {
+            const hipError_t st = hipStreamQuery(s.stream);
+            if (st == hipSuccess) break;
+            if (st != hipErrorNotReady) HIP_CHECK(st);
+            const auto us = std::chrono::duration_cast<std::chrono::microseconds>(
+                                std::chrono::steady_clock::now() - t0).count();
+            if (us > 200) std::this_thread::sleep_for(std::chrono::microseconds(us > 20000 ? 500 : 50));
+        }
+        HIP_CHECK(hipStreamSynchronize(s.stream));
+        float ms = 0.f;
+        HIP_CHECK(hipEventElapsedTime(&ms, s.ev0, s.ev1));
+        const ControlLine *ctl = as<ControlLine>(s.h_ctrl);
+        acc.kernel_ms += ms;
+        acc.donations += ctl[kCtlDonations].v;
+        acc.steals += ctl[kCtlSteals].v;
+        acc.pops += ctl[kCtlPops].v;
+        acc.pushes += ctl[kCtlPushes].v;
+        acc.probes += ctl[kCtlProbes].v;
+        acc.key_reads += ctl[kCtlKeyReads].v;
+        acc.key_writes += ctl[kCtlKeyWrites].v;
+        acc.evictions += ctl[kCtlEvictions].v;
+        acc.spills += ctl[kCtlSpills].v;
+        acc.spill_takes += ctl[kCtlSpillTakes].v;
+        for (int k = 0; k < kNumPhases; ++k) acc.phase_ticks[k] += ctl[kCtlPhase0 + k].v;
+        if (ctl[kCtlStop].v != 0) {
+            stop_flag = (int)ctl[kCtlStop].v;
+            acc.overflow_info = ctl[kCtlOverflowInfo].v;
+        }
+        cancelled = cancel_requested(s);
+        if (stop_flag || cancelled) break;
+    }
+    acc.overflowed = stop_flag != 0;
+    s.stats = acc;
     HIP_CHECK(hipMemcpyAsync(s.h_results.get(), s.results.get(), (size_t)count * sizeof(ConfigResult),
                              hipMemcpyDeviceToHost, s.stream));
-    HIP_CHECK(hipMemcpyAsync(s.h_ctrl.get(), s.ctrl.get(), sizeof(ControlLine) * kNumControl,
-                             hipMemcpyDeviceToHost, s.stream));
-
-    // wait: short spin, then poll with a small sleep; stop polling once a
-    // cancel was requested (the kernel sees the device flag and drains)
-    const auto t0 = std::chrono::steady_clock::now();
-    while (!cancel_requested(s)) {
-        const hipError_t st = hipStreamQuery(s.stream);
-        if (st == hipSuccess) break;
-        if (st != hipErrorNotReady) HIP_CHECK(st);
-        const auto us = std::chrono::duration_cast<std::chrono::microseconds>(
-                            std::chrono::steady_clock::now() - t0).count();
-        if (us > 200) std::this_thread::sleep_for(std::chrono::microseconds(us > 20000 ? 500 : 50));
-    }
     HIP_CHECK(hipStreamSynchronize(s.stream));
-    float ms = 0.f;
-    HIP_CHECK(hipEventElapsedTime(&ms, s.ev0, s.ev1));
-    const ControlLine *ctl = as<ControlLine>(s.h_ctrl);
-    s.stats.kernel_ms = ms;
-    s.stats.waves = waves;
-    s.stats.overflowed = ctl[kCtlStop].v != 0;
-    s.stats.donations = ctl[kCtlDonations].v;
-    s.stats.steals = ctl[kCtlSteals].v;
-    s.stats.node_bytes = node_bytes;
-    s.stats.pops = ctl[kCtlPops].v;
-    s.stats.pushes = ctl[kCtlPushes].v;
-    s.stats.probes = ctl[kCtlProbes].v;
-    s.stats.key_reads = ctl[kCtlKeyReads].v;
-    s.stats.key_writes = ctl[kCtlKeyWrites].v;
-    s.stats.evictions = ctl[kCtlEvictions].v;
-    s.stats.spills = ctl[kCtlSpills].v;
-    s.stats.spill_takes = ctl[kCtlSpillTakes].v;
-    s.stats.key_bytes = (uint64_t)spi_words_for(max_n) * 4;
-    s.stats.overflow_info = ctl[kCtlOverflowInfo].v;
-    for (int k = 0; k < kNumPhases; ++k) s.stats.phase_ticks[k] = ctl[kCtlPhase0 + k].v;
 
     const ConfigResult *res = as<ConfigResult>(s.h_results);
     bool all_found = true;
     for (int c = 0; c < count; ++c) all_found &= res[c].found != 0;
-    const bool cancelled = cancel_requested(s);
-    if (ctl[kCtlStop].v == 2)
+    if (stop_flag == 2)
         throw std::runtime_error("GPU sorting search watchdog timeout (TDP_KERNEL_TIMEOUT_S)");
     // sorting_search.cu:841-846
     if (!all_found && cancelled) throw Cancelled("GPU sorting search cancelled");
-    if (!all_found && ctl[kCtlStop].v != 0) throw QueueOverflow("GPU sorting search queue overflow");
+    if (!all_found && stop_flag != 0) throw QueueOverflow("GPU sorting search queue overflow");
     (void)single_search;
 }
 
@@ -853,9 +876,9 @@ extern "C" {
 
 const char *tdp_last_error(void) { return g_last_error.c_str(); }
 
-int tdp_set_tuning(int waves_per_cu, int share_interval, int share_max, int idle_backoff) {
+int tdp_set_tuning(int waves_per_cu, int share_interval, int share_max, int idle_backoff, int max_active) {
     return guarded([&] {
-        if (waves_per_cu < 0 || share_interval < 0 || share_max < 0 || idle_backoff < 0)
+        if (waves_per_cu < 0 || share_interval < 0 || share_max < 0 || idle_backoff < 0 || max_active < 0)
             throw std::invalid_argument("tuning values must be non-negative");
         if (share_interval && (share_interval & (share_interval - 1)))
             throw std::invalid_argument("share_interval must be a power of two");
@@ -864,6 +887,7 @@ int tdp_set_tuning(int waves_per_cu, int share_interval, int share_max, int idle
         if (share_interval) t.share_interval = share_interval;
         if (share_max) t.share_max = share_max;
         if (idle_backoff) t.idle_backoff = idle_backoff;
+        if (max_active) t.max_active = max_active;
         std::lock_guard<std::mutex> lk(g_tuning_mutex);
         g_tuning = t;
     });

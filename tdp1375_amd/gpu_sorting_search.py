@@ -96,9 +96,10 @@ def filter_last_kernel_ms() -> float:
     return float(_lib.load().tdp_filter_last_kernel_ms())
 
 
-def set_tuning(waves_per_cu: int = 0, share_interval: int = 0, share_max: int = 0, idle_backoff: int = 0) -> None:
+def set_tuning(waves_per_cu: int = 0, share_interval: int = 0, share_max: int = 0, idle_backoff: int = 0,
+               max_active: int = 0) -> None:
     """Work-sharing launch tuning for later launches (0 keeps a default)."""
-    _lib.check(_lib.load().tdp_set_tuning(waves_per_cu, share_interval, share_max, idle_backoff))
+    _lib.check(_lib.load().tdp_set_tuning(waves_per_cu, share_interval, share_max, idle_backoff, max_active))
 
 
 def last_launch_stats(slot: int) -> dict:

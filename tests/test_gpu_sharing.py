@@ -65,3 +65,23 @@ def test_single_configuration_spreads_over_waves(tuned):
     assert st["donations"] > 0 and st["steals"] <= st["donations"]
     assert got.found[0]
     assert_witness(case.pi, case.spi, case.even_cycles, wl.min_rate, wl.max_depth, got.moves[0])
+
+
+def test_batch_runs_in_rounds_of_max_active(tuned):
+    """A batch larger than max_active runs as consecutive launches (fresh dedup
+    epoch each); verdicts and witnesses are those of the whole batch, also when
+    the configurations are re-run from device memory."""
+    wl = W.workload_random_n12()
+    cases = wl.cases[:200]
+    want = _verdicts(cases, wl)
+    T.set_tuning(max_active=48)
+    T.init_slots(1, 1, 1 << 30, 1 << 20)
+    args = ([c.pi for c in cases], [c.spi for c in cases], [c.even_cycles for c in cases], wl.min_rate,
+            wl.max_depth, 0)
+    for flags in (0, T._lib.FLAG_RESIDENT):
+        got = T.search_batch(0, *args, flags=flags)
+        assert [bool(f) for f in got.found] == want
+        for c, case in enumerate(cases):
+            if got.found[c]:
+                assert_witness(case.pi, case.spi, case.even_cycles, wl.min_rate, wl.max_depth, got.moves[c])
+    T.set_tuning()
