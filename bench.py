@@ -1,10 +1,11 @@
 """Benchmark of the MI355X proof-search path (one GPU).
 
-A step is one persistent launch of the plain search over the whole depth-11
-batch: 4,096 configurations, n uniform in 9..36, 3..12 odd cycles of lengths
-3/5/7 (30 % with an oriented 3-cycle), max_ratio 1.375, max_depth 11, dedup
-table of 2^27 entries.  The configurations are uploaded once before the timed
-region and stay resident in HBM (TDP_FLAG_RESIDENT).
+A step is one plain search over the whole depth-11 batch: 4,096
+configurations, n uniform in 9..36, 3..12 odd cycles of lengths 3/5/7 (30 %
+with an oriented 3-cycle), max_ratio 1.375, max_depth 11, dedup table of 2^27
+entries, run by the library as consecutive persistent launches of at most 512
+configurations.  The configurations are uploaded once before the timed region
+and stay resident in HBM (TDP_FLAG_RESIDENT).
 
 Prints one JSON line: states expanded per second (value), cases per hour,
 the search kernel's roofline figure (algorithmic queue + dedup bytes of the
@@ -86,7 +87,7 @@ def cpu_baseline(wl, sample_cases: int, cap: int, threads: int):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--cases", type=int, default=4096)
     ap.add_argument("--queue-mb", type=int, default=32768)
