@@ -87,6 +87,8 @@ typedef struct tdp_launch_stats {
 #define TDP_FLAG_DETERMINISTIC 1u   /* one wave per configuration, no sharing: */
                                     /* counters are reproducible                */
 #define TDP_FLAG_SINGLE_WAVE   2u   /* the whole launch runs on one wave        */
+#define TDP_FLAG_LEARNED       8u   /* order each node's admitted children by the */
+                                    /* move-ordering model (tdp_mlp_*), best first */
 #define TDP_FLAG_RESIDENT      4u   /* tdp_search_batch: reuse the configurations */
                                     /* already in device memory from the previous */
                                     /* batch on this slot (same count); the host  */
@@ -149,6 +151,13 @@ TDP_API double tdp_filter_last_kernel_ms(void);
  * cycle analysis and O(1) parity delta. out: C(n,3) ints. */
 TDP_API int tdp_eval_children(const uint8_t *pi, const uint8_t *spi, int n, int32_t *out);
 
+/* Validation entry (extension): the learned ordering's 128 feature bits of
+ * each of the C(n,3) moves of the node (pi, spi) at depth `depth` of a search
+ * to max_depth with start odd-cycle count e0, as the search kernel computes
+ * them.  out: C(n,3) x 4 words, lexicographic triple order. */
+TDP_API int tdp_eval_features(const uint8_t *pi, const uint8_t *spi, int n, int e0, int depth, int max_depth,
+                              uint32_t *out);
+
 /* Host-only helpers (no GPU): the same parity delta on the CPU, and the
  * predicate tables the kernel uses. */
 TDP_API int tdp_host_eval_children(const uint8_t *pi, const uint8_t *spi, int n, int32_t *out);
@@ -163,6 +172,30 @@ TDP_API int tdp_host_predicate_tables(int n, int initial_even_cycles, float min_
  * launches).  share_interval must be a power of two. */
 TDP_API int tdp_set_tuning(int waves_per_cu, int share_interval, int share_max, int idle_backoff,
                            int max_active);
+
+/* ---- learned move ordering (extension; no reference counterpart) ----
+ * A 128 -> 256 (ReLU) -> 1 MLP scores the candidate moves of a node from 128
+ * feature bits (tdp_learn.h); searches with TDP_FLAG_LEARNED push the admitted
+ * children of every node in ascending score order, so the best-scored child
+ * is taken first.  Verdicts do not depend on the order.  Weights are bf16 on
+ * the device (fp32 master copy for training), one model per process.
+ * Layout: w1[k * 256 + j] (input k, hidden j), b1[256], w2[256], b2.
+ * Feature rows are 4 uint32 words (bit b of the row = word b/32, bit b%32). */
+TDP_API int tdp_mlp_set_weights(const float *w1, const float *b1, const float *w2, float b2);
+TDP_API int tdp_mlp_get_weights(float *w1, float *b1, float *w2, float *b2);
+TDP_API int tdp_mlp_init(uint64_t seed);
+/* logits of N rows */
+TDP_API int tdp_mlp_forward(const uint32_t *rows, int N, float *logits);
+/* Loss and gradients (flat [w1 | b1 | w2 | b2], 33,281 floats) of the softmax
+ * cross-entropy of each group's logits against its one labelled row, mean over
+ * the G groups; group g holds rows group_start[g] .. group_start[g+1]-1. */
+TDP_API int tdp_mlp_gradients(const uint32_t *rows, const uint8_t *labels, const int32_t *group_start, int G,
+                              float *grads, float *loss);
+/* Adam training on the GPU: `epochs` passes over the groups in random order
+ * (seed), batches of groups_per_batch groups; epoch_loss[e] = mean loss of
+ * epoch e (may be NULL). */
+TDP_API int tdp_mlp_train(const uint32_t *rows, const uint8_t *labels, const int32_t *group_start, int G,
+                          int epochs, float lr, int groups_per_batch, uint64_t seed, float *epoch_loss);
 
 /* Number of visible GPUs (0 when none). */
 TDP_API int tdp_device_count(void);

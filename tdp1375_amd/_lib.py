@@ -26,12 +26,14 @@ TDP_ERR_INTERNAL = 6
 FLAG_DETERMINISTIC = 1
 FLAG_SINGLE_WAVE = 2
 FLAG_RESIDENT = 4
+FLAG_LEARNED = 8
 
 EXPORTED_SYMBOLS = (
     "tdp_last_error", "tdp_init_slots", "tdp_destroy_slots", "tdp_search", "tdp_search_ex",
-    "tdp_search_batch", "tdp_last_launch_stats", "tdp_cancel", "tdp_filter",
+    "tdp_search_batch", "tdp_last_launch_stats", "tdp_cancel", "tdp_filter", "tdp_eval_features",
     "tdp_filter_last_kernel_ms", "tdp_eval_children", "tdp_host_eval_children",
-    "tdp_host_predicate_tables", "tdp_set_tuning", "tdp_device_count",
+    "tdp_host_predicate_tables", "tdp_set_tuning", "tdp_device_count", "tdp_mlp_set_weights",
+    "tdp_mlp_get_weights", "tdp_mlp_init", "tdp_mlp_forward", "tdp_mlp_gradients", "tdp_mlp_train",
     "Java_br_unb_cic_tdp_base_GPUSortingSearch_initSlots",
     "Java_br_unb_cic_tdp_base_GPUSortingSearch_destroySlots",
     "Java_br_unb_cic_tdp_base_GPUSortingSearch_searchForSortingSeq",
@@ -116,11 +118,22 @@ def load() -> ctypes.CDLL:
         L.tdp_filter_last_kernel_ms.argtypes = []
         L.tdp_eval_children.argtypes = [u8p, u8p, ctypes.c_int, i32p]
         L.tdp_host_eval_children.argtypes = [u8p, u8p, ctypes.c_int, i32p]
+        L.tdp_eval_features.argtypes = [u8p, u8p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                        ctypes.POINTER(ctypes.c_uint32)]
         L.tdp_host_predicate_tables.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_int,
                                                 ctypes.c_int, ctypes.POINTER(ctypes.c_uint64),
                                                 ctypes.POINTER(ctypes.c_uint64)]
         L.tdp_set_tuning.argtypes = [ctypes.c_int] * 5
         L.tdp_device_count.argtypes = []
+        f32p = ctypes.POINTER(ctypes.c_float)
+        u32p = ctypes.POINTER(ctypes.c_uint32)
+        L.tdp_mlp_set_weights.argtypes = [f32p, f32p, f32p, ctypes.c_float]
+        L.tdp_mlp_get_weights.argtypes = [f32p, f32p, f32p, f32p]
+        L.tdp_mlp_init.argtypes = [ctypes.c_uint64]
+        L.tdp_mlp_forward.argtypes = [u32p, ctypes.c_int, f32p]
+        L.tdp_mlp_gradients.argtypes = [u32p, u8p, i32p, ctypes.c_int, f32p, f32p]
+        L.tdp_mlp_train.argtypes = [u32p, u8p, i32p, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_int,
+                                    ctypes.c_uint64, f32p]
         _lib = L
         return L
 

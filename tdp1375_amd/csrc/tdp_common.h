@@ -173,7 +173,7 @@ enum PhaseIndex {
     kPhaseAnalyse,      // cycle analysis of the node
     kPhaseScan,         // child evaluation over all triples
     kPhaseDedup,        // dedup of staged children
-    kPhasePush,         // pushes of admitted children
+    kPhasePush,         // pushes of admitted children (and their learned scores)
     kPhaseShare,        // donation, idle polling, inbox receipt and pool takes
     kPhaseRoot,         // root expansions
     kNumPhases

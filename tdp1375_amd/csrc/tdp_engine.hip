@@ -28,6 +28,7 @@
 #include "tdp_common.h"
 #include "tdp_filter.h"
 #include "tdp_search.h"
+#include "tdp_learn.h"
 
 namespace tdp {
 namespace {
@@ -281,6 +282,125 @@ DeviceShared &device_shared(int device) {
     return *g_dev[device];
 }
 
+// ---------------------------------------------------------------- learned ordering
+// One move-ordering model per process, on the device that was current when it
+// was first set: fp32 master parameters, Adam moments and the bf16 copies the
+// MFMA passes read (no reference counterpart).
+struct LearnedModel {
+    int device = -1;
+    DevPtr params, grads, m, v, w1t, b1, w2, b2, loss;
+    bool ready = false;
+    int step = 0;
+};
+std::mutex g_model_mutex;
+LearnedModel g_model;
+
+void ensure_model_buffers() {
+    if (g_model.device >= 0) {
+        HIP_CHECK(hipSetDevice(g_model.device));
+        return;
+    }
+    int dev = 0;
+    HIP_CHECK(hipGetDevice(&dev));
+    g_model.device = dev;
+    g_model.params = make_dev(kMlpParams * 4, "mlp.params");
+    g_model.grads = make_dev(kMlpParams * 4, "mlp.grads");
+    g_model.m = make_dev(kMlpParams * 4, "mlp.adam");
+    g_model.v = make_dev(kMlpParams * 4, "mlp.adam");
+    g_model.w1t = make_dev(kIn * kHidden * 2, "mlp.w1");
+    g_model.b1 = make_dev(kHidden * 4, "mlp.b1");
+    g_model.w2 = make_dev(kHidden * 2, "mlp.w2");
+    g_model.b2 = make_dev(4, "mlp.b2");
+    g_model.loss = make_dev(4, "mlp.loss");
+}
+
+MlpWeights model_weights() {
+    MlpWeights M;
+    M.w1t = as<__bf16>(g_model.w1t);
+    M.b1 = as<float>(g_model.b1);
+    M.w2 = as<__bf16>(g_model.w2);
+    M.b2 = as<float>(g_model.b2);
+    return M;
+}
+
+void model_set(const float *w1, const float *b1, const float *w2, float b2) {
+    std::vector<float> p(kMlpParams);
+    std::memcpy(p.data(), w1, sizeof(float) * kIn * kHidden);
+    std::memcpy(p.data() + kIn * kHidden, b1, sizeof(float) * kHidden);
+    std::memcpy(p.data() + kIn * kHidden + kHidden, w2, sizeof(float) * kHidden);
+    p[kMlpParams - 1] = b2;
+    ensure_model_buffers();
+    HIP_CHECK(hipMemcpy(g_model.params.get(), p.data(), kMlpParams * 4, hipMemcpyHostToDevice));
+    HIP_CHECK(hipMemset(g_model.m.get(), 0, kMlpParams * 4));
+    HIP_CHECK(hipMemset(g_model.v.get(), 0, kMlpParams * 4));
+    hipLaunchKernelGGL(mlp_publish_kernel, dim3((kMlpParams + 255) / 256), dim3(256), 0, 0, as<float>(g_model.params),
+                       as<__bf16>(g_model.w1t), as<float>(g_model.b1), as<__bf16>(g_model.w2), as<float>(g_model.b2));
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipDeviceSynchronize());
+    g_model.ready = true;
+    g_model.step = 0;
+}
+
+void require_model() {
+    if (!g_model.ready) throw std::invalid_argument("no move-ordering model is set (tdp_mlp_set_weights)");
+    HIP_CHECK(hipSetDevice(g_model.device));
+}
+
+// Device copies of a row set (features, labels, group starts).
+struct RowSet {
+    DevPtr X, labels, gstart, logits, hpre, dy;
+    int rows = 0, groups = 0;
+};
+
+RowSet upload_rows(const uint32_t *X, const uint8_t *labels, const int32_t *gstart, int N, int G) {
+    RowSet r;
+    r.rows = N;
+    r.groups = G;
+    r.X = make_dev((size_t)std::max(N, 1) * 16, "mlp.rows");
+    HIP_CHECK(hipMemcpy(r.X.get(), X, (size_t)N * 16, hipMemcpyHostToDevice));
+    r.logits = make_dev((size_t)std::max(N, 1) * 4, "mlp.logits");
+    if (labels) {
+        r.labels = make_dev((size_t)std::max(N, 1), "mlp.labels");
+        HIP_CHECK(hipMemcpy(r.labels.get(), labels, (size_t)N, hipMemcpyHostToDevice));
+        r.gstart = make_dev((size_t)(G + 1) * 4, "mlp.groups");
+        HIP_CHECK(hipMemcpy(r.gstart.get(), gstart, (size_t)(G + 1) * 4, hipMemcpyHostToDevice));
+        r.hpre = make_dev((size_t)std::max(N, 1) * kHidden * 4, "mlp.hidden");
+        r.dy = make_dev((size_t)std::max(N, 1) * 4, "mlp.dy");
+    }
+    return r;
+}
+
+// forward + loss + backward over rows [r0, r0 + n) holding groups
+// [g0, g0 + ng) (group starts relative to the set); gradients into g_model.grads
+void model_backward(const RowSet &rs, int r0, int n, int g0, int ng, float scale, float loss_scale) {
+    const MlpWeights M = model_weights();
+    const uint32_t *X = as<uint32_t>(rs.X) + (size_t)r0 * 4;
+    float *hpre = as<float>(rs.hpre) + (size_t)r0 * kHidden;
+    float *logits = as<float>(rs.logits);
+    float *dy = as<float>(rs.dy);
+    const int waves = (n + 15) / 16;
+    hipLaunchKernelGGL(mlp_forward_kernel, dim3((waves + 3) / 4), dim3(256), 0, 0, X, n, M, hpre, logits + r0);
+    hipLaunchKernelGGL(group_softmax_kernel, dim3((ng + 255) / 256), dim3(256), 0, 0, logits,
+                       as<uint8_t>(rs.labels), as<int32_t>(rs.gstart) + g0, ng, scale, loss_scale, dy,
+                       as<float>(g_model.loss));
+    float *g = as<float>(g_model.grads);
+    HIP_CHECK(hipMemsetAsync(g, 0, (size_t)kIn * kHidden * 4, 0));
+    const int rows_per_chunk = 4096;
+    const int chunks = (n + rows_per_chunk - 1) / rows_per_chunk;
+    hipLaunchKernelGGL(mlp_dw1_kernel, dim3(128 * chunks / 4), dim3(256), 0, 0, X, n, hpre, dy + r0, M.w2,
+                       rows_per_chunk, g);
+    hipLaunchKernelGGL(mlp_column_kernel, dim3(kHidden), dim3(256), 0, 0, n, hpre, dy + r0, M.w2,
+                       g + kIn * kHidden, g + kIn * kHidden + kHidden, g + kMlpParams - 1);
+    HIP_CHECK(hipGetLastError());
+}
+
+uint64_t splitmix(uint64_t &s) {
+    uint64_t z = (s += 0x9E3779B97F4A7C15ULL);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+
 // Launch tuning of the work-sharing mode (tdp_set_tuning); 0 = default.
 struct Tuning {
     int waves_per_cu = 0;     // default: the occupancy limit of search_kernel
@@ -488,6 +608,7 @@ unsigned long long kernel_timeout_ticks() {
 void run_launch(Slot &s, const std::vector<CaseInput> &cases, uint32_t flags, bool single_search) {
     const int count = (int)cases.size();
     const bool resident = (flags & TDP_FLAG_RESIDENT) != 0;
+    const bool learned = (flags & TDP_FLAG_LEARNED) != 0;
     HIP_CHECK(hipSetDevice(s.device));
     DeviceShared &dev = device_shared(s.device);
     ensure_config_capacity(s, (size_t)count);
@@ -580,6 +701,12 @@ void run_launch(Slot &s, const std::vector<CaseInput> &cases, uint32_t flags, bo
     P.share_max = share_max;
     P.idle_backoff = (uint32_t)tune.idle_backoff;
     P.timeout_ticks = kernel_timeout_ticks();
+    if (learned) {
+        std::lock_guard<std::mutex> lk(g_model_mutex);
+        if (!g_model.ready) throw std::invalid_argument("no move-ordering model is set (tdp_mlp_set_weights)");
+        if (g_model.device != s.device) throw std::invalid_argument("the move-ordering model is on another device");
+        P.mlp = model_weights();
+    }
 
     if (!resident)
         HIP_CHECK(hipMemcpyAsync(s.cfgs.get(), s.h_cfgs.get(), (size_t)count * sizeof(ConfigDesc),
@@ -617,7 +744,10 @@ void run_launch(Slot &s, const std::vector<CaseInput> &cases, uint32_t flags, bo
         hipLaunchKernelGGL(reset_kernel, dim3((unsigned)((reset_n + 255) / 256)), dim3(256), 0, s.stream, P);
         HIP_CHECK(hipGetLastError());
         HIP_CHECK(hipEventRecord(s.ev0, s.stream));
-        hipLaunchKernelGGL(search_kernel, dim3(blocks), dim3(256), 0, s.stream, P);
+        if (learned)
+            hipLaunchKernelGGL(search_learned_kernel, dim3(blocks), dim3(256), 0, s.stream, P);
+        else
+            hipLaunchKernelGGL(search_kernel, dim3(blocks), dim3(256), 0, s.stream, P);
         HIP_CHECK(hipGetLastError());
         HIP_CHECK(hipEventRecord(s.ev1, s.stream));
         HIP_CHECK(hipMemcpyAsync(s.h_ctrl.get(), s.ctrl.get(), sizeof(ControlLine) * kNumControl,
@@ -1023,10 +1153,32 @@ int tdp_eval_children(const uint8_t *pi, const uint8_t *spi, int n, int32_t *out
         HIP_CHECK(hipMemcpy(d_spi.get(), spi, (size_t)n, hipMemcpyHostToDevice));
         const int e = count_odd_cycles(spi, n);
         hipLaunchKernelGGL(eval_children_kernel, dim3(1), dim3(64), 0, nullptr, as<uint8_t>(d_pi), as<uint8_t>(d_spi),
-                           n, T, e, as<int32_t>(d_out));
+                           n, T, e, as<int32_t>(d_out), 0, 0, 1, nullptr);
         HIP_CHECK(hipGetLastError());
         HIP_CHECK(hipDeviceSynchronize());
         HIP_CHECK(hipMemcpy(out, d_out.get(), (size_t)T * 4, hipMemcpyDeviceToHost));
+    });
+}
+
+int tdp_eval_features(const uint8_t *pi, const uint8_t *spi, int n, int e0, int depth, int max_depth,
+                      uint32_t *out) {
+    return guarded([&] {
+        if (n < 0 || n > kMaxN) throw std::invalid_argument("performGpuSortingSearch: n exceeds MAX_N");
+        if (!out) throw std::invalid_argument("null argument");
+        const int T = triple_count(n);
+        if (T == 0) return;
+        CaseInput c{pi, spi, n, 0, 1, 0, 1.0f};
+        validate_case(c);
+        DevPtr d_pi = make_dev(64, "eval.pi"), d_spi = make_dev(64, "eval.spi");
+        DevPtr d_out = make_dev((size_t)T * 16, "eval.features");
+        HIP_CHECK(hipMemcpy(d_pi.get(), pi, (size_t)n, hipMemcpyHostToDevice));
+        HIP_CHECK(hipMemcpy(d_spi.get(), spi, (size_t)n, hipMemcpyHostToDevice));
+        const int e = count_odd_cycles(spi, n);
+        hipLaunchKernelGGL(eval_children_kernel, dim3(1), dim3(64), 0, nullptr, as<uint8_t>(d_pi), as<uint8_t>(d_spi),
+                           n, T, e, nullptr, e0, depth, max_depth, as<uint32_t>(d_out));
+        HIP_CHECK(hipGetLastError());
+        HIP_CHECK(hipDeviceSynchronize());
+        HIP_CHECK(hipMemcpy(out, d_out.get(), (size_t)T * 16, hipMemcpyDeviceToHost));
     });
 }
 
@@ -1046,6 +1198,129 @@ int tdp_host_predicate_tables(int n, int e0, float min_rate, int max_depth, int 
         if (max_depth <= 0 || max_depth > kMaxDepth)
             throw std::invalid_argument("performGpuSortingSearch: invalid maximumDepth");
         predicate_tables(n, e0, min_rate, max_depth, full != 0, acc, rch);
+    });
+}
+
+int tdp_mlp_set_weights(const float *w1, const float *b1, const float *w2, float b2) {
+    return guarded([&] {
+        if (!w1 || !b1 || !w2) throw std::invalid_argument("null argument");
+        std::lock_guard<std::mutex> lk(g_model_mutex);
+        model_set(w1, b1, w2, b2);
+    });
+}
+
+int tdp_mlp_get_weights(float *w1, float *b1, float *w2, float *b2) {
+    return guarded([&] {
+        if (!w1 || !b1 || !w2 || !b2) throw std::invalid_argument("null argument");
+        std::lock_guard<std::mutex> lk(g_model_mutex);
+        require_model();
+        std::vector<float> p(kMlpParams);
+        HIP_CHECK(hipMemcpy(p.data(), g_model.params.get(), kMlpParams * 4, hipMemcpyDeviceToHost));
+        std::memcpy(w1, p.data(), sizeof(float) * kIn * kHidden);
+        std::memcpy(b1, p.data() + kIn * kHidden, sizeof(float) * kHidden);
+        std::memcpy(w2, p.data() + kIn * kHidden + kHidden, sizeof(float) * kHidden);
+        *b2 = p[kMlpParams - 1];
+    });
+}
+
+int tdp_mlp_init(uint64_t seed) {
+    return guarded([&] {
+        // He-normal first layer, small second layer (Box-Muller on SplitMix64)
+        std::vector<float> w1(kIn * kHidden), b1(kHidden, 0.f), w2(kHidden);
+        uint64_t st = seed;
+        auto normal = [&]() {
+            const double u1 = ((splitmix(st) >> 11) + 1.0) / 9007199254740993.0;
+            const double u2 = (splitmix(st) >> 11) / 9007199254740992.0;
+            return std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
+        };
+        for (auto &x : w1) x = (float)(normal() * std::sqrt(2.0 / 16.0));
+        for (auto &x : w2) x = (float)(normal() * 0.05);
+        std::lock_guard<std::mutex> lk(g_model_mutex);
+        model_set(w1.data(), b1.data(), w2.data(), 0.f);
+    });
+}
+
+int tdp_mlp_forward(const uint32_t *X, int N, float *logits) {
+    return guarded([&] {
+        if (N < 0 || (N > 0 && (!X || !logits))) throw std::invalid_argument("null argument");
+        if (N == 0) return;
+        std::lock_guard<std::mutex> lk(g_model_mutex);
+        require_model();
+        RowSet rs = upload_rows(X, nullptr, nullptr, N, 0);
+        const int waves = (N + 15) / 16;
+        hipLaunchKernelGGL(mlp_forward_kernel, dim3((waves + 3) / 4), dim3(256), 0, 0, as<uint32_t>(rs.X), N,
+                           model_weights(), nullptr, as<float>(rs.logits));
+        HIP_CHECK(hipGetLastError());
+        HIP_CHECK(hipMemcpy(logits, rs.logits.get(), (size_t)N * 4, hipMemcpyDeviceToHost));
+    });
+}
+
+int tdp_mlp_gradients(const uint32_t *X, const uint8_t *labels, const int32_t *group_start, int G, float *grads,
+                      float *loss) {
+    return guarded([&] {
+        if (G <= 0 || !X || !labels || !group_start || !grads || !loss) throw std::invalid_argument("null argument");
+        const int N = group_start[G];
+        if (N <= 0) throw std::invalid_argument("empty row set");
+        std::lock_guard<std::mutex> lk(g_model_mutex);
+        require_model();
+        RowSet rs = upload_rows(X, labels, group_start, N, G);
+        HIP_CHECK(hipMemset(g_model.loss.get(), 0, 4));
+        model_backward(rs, 0, N, 0, G, 1.0f / (float)G, 1.0f / (float)G);
+        HIP_CHECK(hipDeviceSynchronize());
+        HIP_CHECK(hipMemcpy(grads, g_model.grads.get(), kMlpParams * 4, hipMemcpyDeviceToHost));
+        HIP_CHECK(hipMemcpy(loss, g_model.loss.get(), 4, hipMemcpyDeviceToHost));
+    });
+}
+
+int tdp_mlp_train(const uint32_t *X, const uint8_t *labels, const int32_t *group_start, int G, int epochs,
+                  float lr, int groups_per_batch, uint64_t seed, float *epoch_loss) {
+    return guarded([&] {
+        if (G <= 0 || epochs <= 0 || groups_per_batch <= 0 || !X || !labels || !group_start)
+            throw std::invalid_argument("invalid training arguments");
+        const int N = group_start[G];
+        std::lock_guard<std::mutex> lk(g_model_mutex);
+        require_model();
+        std::vector<int> order(G);
+        for (int g = 0; g < G; ++g) order[g] = g;
+        std::vector<uint32_t> Xp((size_t)N * 4);
+        std::vector<uint8_t> Lp(N);
+        std::vector<int32_t> Gp(G + 1);
+        uint64_t st = seed;
+        for (int ep = 0; ep < epochs; ++ep) {
+            // groups in a new random order each epoch (Fisher-Yates)
+            for (int g = G - 1; g > 0; --g) std::swap(order[g], order[(int)(splitmix(st) % (uint64_t)(g + 1))]);
+            int r = 0;
+            for (int q = 0; q < G; ++q) {
+                const int g = order[q];
+                Gp[q] = r;
+                for (int x = group_start[g]; x < group_start[g + 1]; ++x, ++r) {
+                    std::memcpy(&Xp[(size_t)r * 4], &X[(size_t)x * 4], 16);
+                    Lp[r] = labels[x];
+                }
+            }
+            Gp[G] = r;
+            RowSet rs = upload_rows(Xp.data(), Lp.data(), Gp.data(), N, G);
+            HIP_CHECK(hipMemset(g_model.loss.get(), 0, 4));
+            for (int g0 = 0; g0 < G; g0 += groups_per_batch) {
+                const int ng = std::min(groups_per_batch, G - g0);
+                const int r0 = Gp[g0], n = Gp[g0 + ng] - r0;
+                if (n <= 0) continue;
+                // gradients: mean over the batch's groups; loss: mean over the epoch
+                model_backward(rs, r0, n, g0, ng, 1.0f / (float)ng, 1.0f / (float)G);
+                g_model.step++;
+                const float b1c = 1.0f - std::pow(0.9f, (float)g_model.step);
+                const float b2c = 1.0f - std::pow(0.999f, (float)g_model.step);
+                hipLaunchKernelGGL(mlp_adam_kernel, dim3((kMlpParams + 255) / 256), dim3(256), 0, 0,
+                                   as<float>(g_model.params), as<float>(g_model.grads), as<float>(g_model.m),
+                                   as<float>(g_model.v), kMlpParams, lr, b1c, b2c, as<__bf16>(g_model.w1t),
+                                   as<float>(g_model.b1), as<__bf16>(g_model.w2), as<float>(g_model.b2));
+                HIP_CHECK(hipGetLastError());
+            }
+            HIP_CHECK(hipDeviceSynchronize());
+            float l = 0.f;
+            HIP_CHECK(hipMemcpy(&l, g_model.loss.get(), 4, hipMemcpyDeviceToHost));
+            if (epoch_loss) epoch_loss[ep] = l;
+        }
     });
 }
 

@@ -5,6 +5,7 @@
 #include <stdint.h>
 
 #include "tdp_common.h"
+#include "tdp_learn.h"
 
 namespace tdp {
 
@@ -48,11 +49,13 @@ struct SearchParams {
     uint32_t share_max;           // nodes handed over per donation
     uint32_t idle_backoff;        // longest idle sleep, in units of s_sleep 127
     unsigned long long timeout_ticks;  // watchdog: 100 MHz ticks per wave, 0 = none
+    MlpWeights mlp;               // learned ordering (search_learned_kernel only)
 };
 
 __global__ void search_kernel(SearchParams P);
+__global__ void search_learned_kernel(SearchParams P);
 __global__ void reset_kernel(SearchParams P);
 __global__ void eval_children_kernel(const uint8_t *pi, const uint8_t *spi, int n,
-                                     int T, int e, int32_t *out);
+                                     int T, int e, int32_t *out, int e0, int d, int D, uint32_t *features);
 
 }  // namespace tdp

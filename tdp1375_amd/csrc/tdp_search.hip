@@ -31,6 +31,7 @@
 
 #include "tdp_common.h"
 #include "tdp_search.h"
+#include "tdp_learn.h"
 
 namespace tdp {
 namespace {
@@ -53,6 +54,7 @@ struct WaveLds {
     uint8_t posof[64];                       // position of each symbol in pi
     uint32_t cyc[24];                        // per cycle: first candidate | len << 16 | member offset << 24
     uint32_t stage[kStage];
+    uint8_t rowsrc[64];                      // learned ordering: lane of each compacted admitted child
     uint32_t ckey[16][kWave];                // child spi words of each lane's staged child
     unsigned long long cnt[kNumCounters];    // counters of the current configuration
     unsigned long long pend[kNumCounters];   // counters of the current expansion
@@ -749,6 +751,7 @@ __device__ __forceinline__ int dedup_admit(const SearchParams &P, WaveLds &L, in
 
 // Deduplicates and pushes the staged children (descending triple order).
 // root_stage: children of the start node (not counted as admitted).
+template <bool kLearned>
 __device__ __forceinline__ void flush_stage(const SearchParams &P, WaveState &W, WaveLds &L, int count, int n, int sw,
                             int T, bool root_stage) {
     const uint32_t cfg = L.node[0];
@@ -787,11 +790,64 @@ __device__ __forceinline__ void flush_stage(const SearchParams &P, WaveState &W,
             lds_add(&L.pend[kCntAdmitted], imp + ins + uni);
         }
         unsigned long long pm = __ballot(admit);
+        float score = 0.f;
+        if (kLearned && pm) {
+            // learned ordering: score the admitted children (MFMA MLP over
+            // tiles of 16 of them, compacted in lane order)
+            PHASE_T0();
+            const ConfigDesc &C = P.cfgs[cfg];
+            const int e = (int)((L.node[1] >> 8) & 0xFF);
+            uint32_t f[kFeatWords];
+            move_features(L.posinfo[i], L.posinfo[j], L.posinfo[k], i, j, k, n, d, (int)C.max_depth,
+                          (int)C.e_start, ce, ce - e, f);
+            const unsigned long long below = (W.lane == 0) ? 0ULL : (pm & (~0ULL >> (64 - W.lane)));
+            const int ci = __popcll(below);
+            const int cnt = __popcll(pm);
+            if (admit) L.rowsrc[ci] = (uint8_t)W.lane;
+            wave_sync();
+            for (int rt = 0; rt * 16 < cnt; ++rt) {
+                const int row = rt * 16 + (W.lane & 15);
+                const int src = row < cnt ? (int)L.rowsrc[row] : 0;
+                uint32_t fw[kFeatWords];
+#pragma unroll
+                for (int w = 0; w < kFeatWords; ++w) {
+                    const uint32_t v = (uint32_t)__shfl((int)f[w], src);
+                    fw[w] = row < cnt ? v : 0u;
+                }
+                float out[4];
+                mlp_tile_logits(P.mlp, fw, W.lane, out);
+                const int local = ci - rt * 16;
+                const int q = (local >> 2) & 3, r = local & 3;
+                const float s0 = __shfl(out[0], q * 16), s1 = __shfl(out[1], q * 16);
+                const float s2 = __shfl(out[2], q * 16), s3 = __shfl(out[3], q * 16);
+                if (admit && local >= 0 && local < 16) score = r == 0 ? s0 : r == 1 ? s1 : r == 2 ? s2 : s3;
+            }
+            PHASE_ADD(L, kPhasePush);
+        }
         {
             PHASE_T0();
             while (pm) {
-                const int src = __ffsll((long long)pm) - 1;
-                pm &= pm - 1;
+                int src;
+                if (kLearned) {
+                    // ascending score: the best-scored child is pushed last
+                    // and so taken first (ties: lower lane first)
+                    float key = ((pm >> W.lane) & 1ULL) ? score : 3.0e38f;
+                    int best = W.lane;
+#pragma unroll
+                    for (int off = 32; off > 0; off >>= 1) {
+                        const float ok = __shfl_xor(key, off);
+                        const int ol = __shfl_xor(best, off);
+                        if (ok < key || (ok == key && ol < best)) {
+                            key = ok;
+                            best = ol;
+                        }
+                    }
+                    src = uniform(best);
+                    pm &= ~(1ULL << src);
+                } else {
+                    src = __ffsll((long long)pm) - 1;
+                    pm &= pm - 1;
+                }
                 const int ct = __shfl(t, src), ci = __shfl(i, src), cj = __shfl(j, src), ck = __shfl(k, src);
                 const int cce = __shfl(ce, src);
                 push_child(P, W, L, n, sw, uniform(ct), uniform(ci), uniform(cj), uniform(ck), uniform(cce));
@@ -927,6 +983,7 @@ __device__ __forceinline__ bool cycle_local_children(WaveState &W, WaveLds &L, u
 // Evaluates every child of L.node.  Normal mode: returns the goal triple
 // (terminal expansion) or -1 after staging/pushing admissible children.
 // Root mode: stages and pushes every live root.
+template <bool kLearned>
 __device__ __forceinline__ int expand(const SearchParams &P, WaveState &W, WaveLds &L, const ConfigDesc &C,
                       bool root_mode) {
     const int n = (int)C.n;
@@ -1007,7 +1064,7 @@ __device__ __forceinline__ int expand(const SearchParams &P, WaveState &W, WaveL
         while (scan_children(W.colex, n, T, W.lane, mb, qb, visit) == kScanPause) {
             // stage full: flush, then resume the scan at the next chunk
             wave_sync();
-            flush_stage(P, W, L, scount, n, sw, T, root_mode);
+            flush_stage<kLearned>(P, W, L, scount, n, sw, T, root_mode);
             scount = 0;
             if (W.stop) return -1;
         }
@@ -1017,7 +1074,7 @@ __device__ __forceinline__ int expand(const SearchParams &P, WaveState &W, WaveL
     if (goal_t >= 0) return goal_t;
     if (scount) {
         wave_sync();
-        flush_stage(P, W, L, scount, n, sw, T, root_mode);
+        flush_stage<kLearned>(P, W, L, scount, n, sw, T, root_mode);
     }
     if (!root_mode) {
         wave_sync();
@@ -1051,11 +1108,8 @@ __device__ __forceinline__ bool should_stop(const SearchParams &P) {
 
 }  // namespace
 
-// Everything is inlined into the kernel: a call would pass the by-value
-// parameter block through scratch memory and LDS references as 64-bit flat
-// pointers.  Three waves per SIMD (<= 168 VGPRs) costs a few spilled
-// registers; four would spill heavily.
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void search_kernel(SearchParams P) {
+template <bool kLearned>
+__device__ __forceinline__ void search_body(const SearchParams &P) {
     __shared__ WaveLds lds[kWavesPerBlock];
     __shared__ uint16_t colex[kColexPairs];
     fill_colex(colex);
@@ -1122,7 +1176,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void s
             wave_sync();
             if (C.triple_count > 0) {
                 PHASE_T0();
-                expand(P, W, L, C, true);
+                expand<kLearned>(P, W, L, C, true);
                 PHASE_ADD(L, kPhaseRoot);
             }
             continue;
@@ -1222,7 +1276,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void s
         if (d >= (int)C.max_depth || !((C.reach[d] >> e) & 1ULL)) continue;
         lds_add(&L.cnt[kCntExpansions], 1);
         lds_add(&L.cnt[kCntGenerated], C.triple_count);
-        const int goal = expand(P, W, L, C, false);
+        const int goal = expand<kLearned>(P, W, L, C, false);
         if (goal >= 0) report_found(P, W, L, cfg, d + 1, goal);
 
         // donation: every share_mask + 1 expansions, a wave with at least two
@@ -1249,6 +1303,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void s
     }
 }
 
+// Everything is inlined into the kernels: a call would pass the by-value
+// parameter block through scratch memory and LDS references as 64-bit flat
+// pointers.  Three waves per SIMD (<= 168 VGPRs) costs a few spilled
+// registers; four would spill heavily.  The learned-ordering variant is a
+// kernel of its own so that the plain search carries no MFMA code.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void search_kernel(SearchParams P) {
+    search_body<false>(P);
+}
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void search_learned_kernel(SearchParams P) {
+    search_body<true>(P);
+}
+
 // Reset of the launch state (replaces the per-search memsets of
 // sorting_search.cu:744-777 with one small kernel).
 __global__ void reset_kernel(SearchParams P) {
@@ -1269,7 +1335,8 @@ __global__ void reset_kernel(SearchParams P) {
 // Debug/validation kernel: odd-cycle count of every child of one node,
 // computed with the same LDS cycle analysis and parity delta as the search.
 __global__ __launch_bounds__(64) void eval_children_kernel(const uint8_t *pi, const uint8_t *spi, int n, int T,
-                                                           int e, int32_t *out) {
+                                                           int e, int32_t *out, int e0, int d, int D,
+                                                           uint32_t *features) {
     __shared__ WaveLds L;
     __shared__ uint16_t colex[kColexPairs];
     fill_colex(colex);
@@ -1295,7 +1362,14 @@ __global__ __launch_bounds__(64) void eval_children_kernel(const uint8_t *pi, co
     int mb = -1, qb = 0;
     scan_children(colex, n, T, lane, mb, qb, [&](bool valid, int i, int j, int k, int obase) -> int {
         const int delta = child_delta(L.posinfo[i], L.posinfo[j], L.posinfo[k]);
-        if (valid) out[T - 1 - (obase + lane)] = e + delta;
+        const int t = T - 1 - (obase + lane);
+        if (valid && out) out[t] = e + delta;
+        if (valid && features) {
+            // the learned ordering's features of this move, as the search computes them
+            uint32_t f[kFeatWords];
+            move_features(L.posinfo[i], L.posinfo[j], L.posinfo[k], i, j, k, n, d, D, e0, e + delta, delta, f);
+            for (int w = 0; w < kFeatWords; ++w) features[(size_t)t * kFeatWords + w] = f[w];
+        }
         return kScanGo;
     });
 }
