@@ -96,6 +96,9 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=192)
     ap.add_argument("--cpu-cap", type=int, default=40000)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB")
+    ap.add_argument("--ordering", choices=("plain", "learned"), default="plain",
+                    help="learned: order children by the trained MLP in --model")
+    ap.add_argument("--model", default=os.path.join(ROOT, "tdp1375_amd", "data", "ordering_mlp_d11.npz"))
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -123,9 +126,15 @@ def main():
     spis = [c.spi for c in cases]
     e0s = [c.even_cycles for c in cases]
 
+    order_flag = 0
+    if args.ordering == "learned":
+        from tdp1375_amd import train_ordering
+        train_ordering.load(args.model)
+        order_flag = T._lib.FLAG_LEARNED
+
     def step(resident: bool):
         return T.search_batch(0, pis, spis, e0s, wl.min_rate, wl.max_depth, 0,
-                              flags=T._lib.FLAG_RESIDENT if resident else 0)
+                              flags=(T._lib.FLAG_RESIDENT if resident else 0) | order_flag)
 
     first = None
     for w in range(max(1, args.warmup)):
@@ -173,10 +182,11 @@ def main():
         "data": "synthetic",
         "config": {"workload": wl.name, "cases": len(cases), "n_range": [9, 36], "max_ratio": wl.max_ratio,
                    "max_depth": wl.max_depth, "dedup_table_size": 1 << args.dedup_log2,
-                   "queue_mb": args.queue_mb, "ordering": "plain", "found": int(verdicts.sum())},
+                   "queue_mb": args.queue_mb, "ordering": args.ordering, "found": int(verdicts.sum())},
         "cases_per_hour": cases_per_hour,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(wl.name),
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(wl.name) if args.ordering == "plain" else None,
                      "kernel": "search_kernel", "avg_kernel_ms": 1e3 * avg_kernel_s,
                      "algorithmic_bytes_per_launch": sum(traffic) / len(traffic)},
     }

@@ -791,9 +791,10 @@ __device__ __forceinline__ void flush_stage(const SearchParams &P, WaveState &W,
         }
         unsigned long long pm = __ballot(admit);
         float score = 0.f;
-        if (kLearned && pm) {
-            // learned ordering: score the admitted children (MFMA MLP over
-            // tiles of 16 of them, compacted in lane order)
+        if (kLearned && (pm & (pm - 1))) {
+            // learned ordering: score the admitted children when there are at
+            // least two (MFMA MLP over tiles of 16 of them, compacted in lane
+            // order)
             PHASE_T0();
             const ConfigDesc &C = P.cfgs[cfg];
             const int e = (int)((L.node[1] >> 8) & 0xFF);
