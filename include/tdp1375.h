@@ -54,7 +54,8 @@ typedef struct tdp_counters {
 /* Result of one configuration of a batch. */
 typedef struct tdp_case_result {
     int32_t found;                       /* 1 found, 0 not found              */
-    int32_t depth;                       /* moves in the sequence             */
+    int32_t depth;                       /* moves in the sequence; -1: undecided */
+                                         /* (TDP_FLAG_SOFT_DEADLINE)            */
     int32_t moves[TDP_MAX_DEPTH][3];     /* positions i<j<k, in move order    */
     tdp_counters counters;
 } tdp_case_result;
@@ -87,6 +88,10 @@ typedef struct tdp_launch_stats {
 #define TDP_FLAG_DETERMINISTIC 1u   /* one wave per configuration, no sharing: */
                                     /* counters are reproducible                */
 #define TDP_FLAG_SINGLE_WAVE   2u   /* the whole launch runs on one wave        */
+#define TDP_FLAG_SOFT_DEADLINE 16u  /* tdp_search_batch: each launch stops after the */
+                                    /* tdp_set_launch_deadline budget without an    */
+                                    /* error; configurations not found by then get  */
+                                    /* depth -1 (undecided) instead of 0            */
 #define TDP_FLAG_LEARNED       8u   /* order each node's admitted children by the */
                                     /* move-ordering model (tdp_mlp_*), best first */
 #define TDP_FLAG_RESIDENT      4u   /* tdp_search_batch: reuse the configurations */
@@ -196,6 +201,10 @@ TDP_API int tdp_mlp_gradients(const uint32_t *rows, const uint8_t *labels, const
  * epoch e (may be NULL). */
 TDP_API int tdp_mlp_train(const uint32_t *rows, const uint8_t *labels, const int32_t *group_start, int G,
                           int epochs, float lr, int groups_per_batch, uint64_t seed, float *epoch_loss);
+
+/* Time budget per persistent launch for TDP_FLAG_SOFT_DEADLINE (extension;
+ * default 1000 ms). */
+TDP_API int tdp_set_launch_deadline(int milliseconds);
 
 /* Number of visible GPUs (0 when none). */
 TDP_API int tdp_device_count(void);

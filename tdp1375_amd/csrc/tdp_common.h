@@ -113,6 +113,10 @@ struct alignas(16) ConfigResult {
     int32_t pad[2];
     uint32_t moves[16];  // triple indices of the solution, in move order
     unsigned long long counters[kNumCounters];
+    // nodes of the configuration pushed minus nodes taken (mod 2^64): zero
+    // once its search space is exhausted, whatever stopped the launch
+    unsigned long long live;
+    unsigned long long pad2;
 };
 
 // Node layout in HBM (32-bit words):

@@ -408,6 +408,7 @@ struct Tuning {
     int share_max = 4;        // nodes per donation
     int idle_backoff = 16;    // longest idle sleep, x 127 s_sleep units
     int max_active = 512;     // configurations per persistent launch
+    int deadline_ms = 1000;   // TDP_FLAG_SOFT_DEADLINE: time budget of each launch
 };
 std::mutex g_tuning_mutex;
 Tuning g_tuning;
@@ -459,6 +460,7 @@ struct Slot {
     int last_batch_count = -1;
     std::vector<CaseInput> last_cases;
     std::vector<int> last_index;
+    std::vector<uint8_t> undecided;   // per case of the last launch (soft deadline)
 };
 
 std::mutex g_slots_mutex;
@@ -609,6 +611,8 @@ void run_launch(Slot &s, const std::vector<CaseInput> &cases, uint32_t flags, bo
     const int count = (int)cases.size();
     const bool resident = (flags & TDP_FLAG_RESIDENT) != 0;
     const bool learned = (flags & TDP_FLAG_LEARNED) != 0;
+    const bool soft = (flags & TDP_FLAG_SOFT_DEADLINE) != 0;
+    s.undecided.assign((size_t)count, 0);
     HIP_CHECK(hipSetDevice(s.device));
     DeviceShared &dev = device_shared(s.device);
     ensure_config_capacity(s, (size_t)count);
@@ -700,7 +704,7 @@ void run_launch(Slot &s, const std::vector<CaseInput> &cases, uint32_t flags, bo
     P.share_mask = (uint32_t)tune.share_interval - 1;
     P.share_max = share_max;
     P.idle_backoff = (uint32_t)tune.idle_backoff;
-    P.timeout_ticks = kernel_timeout_ticks();
+    P.timeout_ticks = soft ? (unsigned long long)tune.deadline_ms * 100000ULL : kernel_timeout_ticks();
     if (learned) {
         std::lock_guard<std::mutex> lk(g_model_mutex);
         if (!g_model.ready) throw std::invalid_argument("no move-ordering model is set (tdp_mlp_set_weights)");
@@ -780,7 +784,15 @@ void run_launch(Slot &s, const std::vector<CaseInput> &cases, uint32_t flags, bo
         acc.spills += ctl[kCtlSpills].v;
         acc.spill_takes += ctl[kCtlSpillTakes].v;
         for (int k = 0; k < kNumPhases; ++k) acc.phase_ticks[k] += ctl[kCtlPhase0 + k].v;
-        if (ctl[kCtlStop].v != 0) {
+        if (soft && ctl[kCtlStop].v == 2) {
+            // the launch's time budget ran out: what is not found is
+            // undecided, and the next round runs with a fresh budget
+            HIP_CHECK(hipMemcpyAsync(s.h_results.get(), s.results.get(), (size_t)count * sizeof(ConfigResult),
+                                     hipMemcpyDeviceToHost, s.stream));
+            HIP_CHECK(hipStreamSynchronize(s.stream));
+            const ConfigResult *r = as<ConfigResult>(s.h_results);
+            for (int c = base; c < base + sub; ++c) s.undecided[c] = (!r[c].found && r[c].live != 0) ? 1 : 0;
+        } else if (ctl[kCtlStop].v != 0) {
             stop_flag = (int)ctl[kCtlStop].v;
             acc.overflow_info = ctl[kCtlOverflowInfo].v;
         }
@@ -1006,6 +1018,14 @@ extern "C" {
 
 const char *tdp_last_error(void) { return g_last_error.c_str(); }
 
+int tdp_set_launch_deadline(int milliseconds) {
+    return guarded([&] {
+        if (milliseconds <= 0) throw std::invalid_argument("deadline must be positive");
+        std::lock_guard<std::mutex> lk(g_tuning_mutex);
+        g_tuning.deadline_ms = milliseconds;
+    });
+}
+
 int tdp_set_tuning(int waves_per_cu, int share_interval, int share_max, int idle_backoff, int max_active) {
     return guarded([&] {
         if (waves_per_cu < 0 || share_interval < 0 || share_max < 0 || idle_backoff < 0 || max_active < 0)
@@ -1013,6 +1033,7 @@ int tdp_set_tuning(int waves_per_cu, int share_interval, int share_max, int idle
         if (share_interval && (share_interval & (share_interval - 1)))
             throw std::invalid_argument("share_interval must be a power of two");
         Tuning t;
+        t.deadline_ms = current_tuning().deadline_ms;
         if (waves_per_cu) t.waves_per_cu = waves_per_cu;
         if (share_interval) t.share_interval = share_interval;
         if (share_max) t.share_max = share_max;
@@ -1097,7 +1118,7 @@ int tdp_search_batch(int slot, int count, const uint8_t *pis, const uint8_t *spi
         for (size_t q = 0; q < cases.size(); ++q) {
             tdp_case_result &o = out[index[q]];
             o.found = res[q].found ? 1 : 0;
-            o.depth = res[q].found ? res[q].depth : 0;
+            o.depth = res[q].found ? res[q].depth : (s.undecided[q] ? -1 : 0);
             for (int m = 0; m < o.depth; ++m) triples().unpack(cases[q].n, res[q].moves[m], &o.moves[m][0]);
             copy_counters(res[q], &o.counters);
         }

@@ -58,6 +58,7 @@ struct WaveLds {
     uint32_t ckey[16][kWave];                // child spi words of each lane's staged child
     unsigned long long cnt[kNumCounters];    // counters of the current configuration
     unsigned long long pend[kNumCounters];   // counters of the current expansion
+    unsigned long long live;                 // pushes - pops of the current configuration
     unsigned long long traffic[6];           // pops, pushes, tag reads, key compares, key writes,
                                              // evictions
     unsigned long long phase[kNumPhases];    // 10 ns ticks per phase (TDP_PHASE_TIMING builds)
@@ -331,7 +332,9 @@ __device__ __forceinline__ void flush_counters(const SearchParams &P, WaveState 
         const unsigned long long v = L.cnt[W.lane];
         if (v) fetch_add(&P.results[W.cur_cfg].counters[W.lane], v);
     }
+    if (W.cur_cfg >= 0 && W.lane == kNumCounters && L.live) fetch_add(&P.results[W.cur_cfg].live, L.live);
     if (W.lane < kNumCounters) L.cnt[W.lane] = 0;
+    if (W.lane == kNumCounters) L.live = 0;
     wave_sync();
 }
 
@@ -583,6 +586,7 @@ __device__ __forceinline__ void push_child(const SearchParams &P, WaveState &W, 
     }
     W.hi++;
     lds_add(&L.traffic[1], 1);
+    lds_add(&L.live, 1ULL);
 }
 
 // Lane-parallel dedup of staged children (sorting_search.cu:261-298
@@ -1127,6 +1131,7 @@ __device__ __forceinline__ void search_body(const SearchParams &P) {
     W.stop = false;
     W.deadline = P.timeout_ticks ? __builtin_amdgcn_s_memrealtime() + P.timeout_ticks : 0ULL;
     if (W.lane < kNumCounters) L.cnt[W.lane] = 0;
+    if (W.lane == 0) L.live = 0;
     if (W.lane < 6) L.traffic[W.lane] = 0;
     if (W.lane < kNumPhases) L.phase[W.lane] = 0;
     wave_sync();
@@ -1266,6 +1271,7 @@ __device__ __forceinline__ void search_body(const SearchParams &P) {
         {
             PHASE_T0();
             switch_config(P, W, L, cfg);
+            lds_add(&L.live, ~0ULL);   // the node just taken
             const bool skip = config_found(P, cfg);
             PHASE_ADD(L, kPhaseCheck);
             if (skip) continue;
@@ -1327,6 +1333,7 @@ __global__ void reset_kernel(SearchParams P) {
         r.depth = 0;
         for (int m = 0; m < 16; ++m) r.moves[m] = 0;
         for (int c = 0; c < kNumCounters; ++c) r.counters[c] = 0;
+        r.live = 0;
     }
     if (tid < (size_t)P.num_waves && P.inbox) P.inbox[tid * 16] = 0;
     if (tid < (size_t)P.idle_words && P.idle_bits) P.idle_bits[tid * 16] = 0;

@@ -102,6 +102,11 @@ def set_tuning(waves_per_cu: int = 0, share_interval: int = 0, share_max: int = 
     _lib.check(_lib.load().tdp_set_tuning(waves_per_cu, share_interval, share_max, idle_backoff, max_active))
 
 
+def set_launch_deadline(milliseconds: int) -> None:
+    """Time budget of each launch run with FLAG_SOFT_DEADLINE."""
+    _lib.check(_lib.load().tdp_set_launch_deadline(int(milliseconds)))
+
+
 def last_launch_stats(slot: int) -> dict:
     s = _lib.LaunchStats()
     _lib.check(_lib.load().tdp_last_launch_stats(slot, ctypes.byref(s)))
@@ -147,7 +152,8 @@ def search_batch(slot: int, pis: Sequence[Sequence[int]], spis: Sequence[Sequenc
     _lib.check(st)
     found = np.array([bool(out[c].found) for c in range(count)])
     depth = np.array([int(out[c].depth) for c in range(count)], np.int32)
-    moves = [[(out[c].moves[3 * m], out[c].moves[3 * m + 1], out[c].moves[3 * m + 2]) for m in range(out[c].depth)]
+    moves = [[(out[c].moves[3 * m], out[c].moves[3 * m + 1], out[c].moves[3 * m + 2])
+              for m in range(max(out[c].depth, 0))]
              for c in range(count)]
     counters = {name: np.array([getattr(out[c].counters, name) for c in range(count)], np.uint64)
                 for name, _ in _lib.Counters._fields_}
