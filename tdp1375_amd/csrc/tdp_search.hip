@@ -50,9 +50,11 @@ enum DedupOutcome { kDedupPending = -1, kDedupDominated = 0, kDedupImproved = 1,
 struct WaveLds {
     uint32_t node[48];
     uint32_t posinfo[64];
-    uint8_t memb[64];                        // cycle members in spi order, cycles of length >= 3
+    uint8_t memb[64];                        // cycle members in spi order, cycles of length >= 2
+    uint8_t ememb[64];                       // the members of the even cycles again, contiguous
     uint8_t posof[64];                       // position of each symbol in pi
-    uint32_t cyc[24];                        // per cycle: first candidate | len << 16 | member offset << 24
+    uint32_t cyc[24];                        // cycles of length >= 3: first candidate | len << 16 | memb offset << 24
+    uint32_t ecyc[32];                       // even cycles: first candidate | len << 16 | ememb offset << 24
     uint32_t stage[kStage];
     uint8_t rowsrc[64];                      // learned ordering: lane of each compacted admitted child
     uint32_t ckey[16][kWave];                // child spi words of each lane's staged child
@@ -85,6 +87,28 @@ __device__ __forceinline__ void wave_sync() {
 }
 
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// Inclusive prefix sum / running maximum over the 64 lanes with DPP row
+// shifts and row broadcasts (a few VALU cycles each, instead of LDS-latency
+// cross-lane shuffles); lane 63 holds the total / maximum.
+__device__ __forceinline__ int wave_scan_add(int v) {
+    v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xF, 0xF, false);   // row_shr:1
+    v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xF, 0xF, false);   // row_shr:2
+    v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xF, 0xF, false);   // row_shr:4
+    v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xF, 0xF, false);   // row_shr:8
+    v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xA, 0xF, false);   // row_bcast:15
+    v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xC, 0xF, false);   // row_bcast:31
+    return v;
+}
+__device__ __forceinline__ int wave_max(int v) {
+    v = max(v, __builtin_amdgcn_update_dpp(-1, v, 0x111, 0xF, 0xF, false));
+    v = max(v, __builtin_amdgcn_update_dpp(-1, v, 0x112, 0xF, 0xF, false));
+    v = max(v, __builtin_amdgcn_update_dpp(-1, v, 0x114, 0xF, 0xF, false));
+    v = max(v, __builtin_amdgcn_update_dpp(-1, v, 0x118, 0xF, 0xF, false));
+    v = max(v, __builtin_amdgcn_update_dpp(-1, v, 0x142, 0xA, 0xF, false));
+    v = max(v, __builtin_amdgcn_update_dpp(-1, v, 0x143, 0xC, 0xF, false));
+    return __builtin_amdgcn_readlane(v, 63);
+}
 
 __device__ __forceinline__ unsigned long long ld_relaxed(const unsigned long long *p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -605,11 +629,7 @@ struct LaneTraffic {
     int probes, key_reads, key_writes, evictions;
 };
 
-__device__ __forceinline__ int wave_sum(int v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-    return v;
-}
+__device__ __forceinline__ int wave_sum(int v) { return __builtin_amdgcn_readlane(wave_scan_add(v), 63); }
 
 // t[q] for a wave-varying q without dynamic register indexing (which would
 // put the array in scratch memory)
@@ -887,67 +907,105 @@ __device__ __forceinline__ void report_found(const SearchParams &P, WaveState &W
     }
 }
 
-// Children of an expansion that needs a gain of +2 on a node whose cycles
-// all have odd length.  Such a child must take its three symbols from one
-// cycle: three different cycles merge into one (delta -2 for odd lengths) and
-// two symbols of one odd cycle with one of another odd cycle leave two cycles
-// of lengths of equal parity (delta <= 0).  So instead of all C(n,3) triples
-// only the sum of C(L,3) over the cycles is evaluated, with the same parity
-// delta as the full scan.  The result is the one of the full scan: the goal is
-// the largest goal triple index, and the admitted children are staged in
-// descending triple order.  Returns false (nothing done) when the node has an
-// even cycle or the candidates are too many for the stage; otherwise sets
+// Children of an expansion that needs a gain of +2.  Such a child either takes
+// its three symbols from one cycle, or two symbols from one even cycle and the
+// third from another even cycle: three different cycles merge into one
+// (delta <= 0), and two symbols of a cycle of length L with one of a cycle of
+// length L' leave two cycles whose odd count can only rise by 2 when L and L'
+// are both even.  So instead of all C(n,3) triples only
+//   sum over cycles C(L,3)  +  sum over even cycles C(L,2) (E - L)
+// triples are evaluated (E: symbols in even cycles), with the same parity
+// delta as the full scan, and the result is the full scan's: the goal is the
+// largest goal triple index, admitted children are staged in descending
+// triple order.  Returns false (nothing done) when the candidates are too
+// many for the stage or not clearly fewer than C(n,3); otherwise sets
 // scount / goal_t as the scan would.
 constexpr int kCycleLocalMax = 512;
 
-__device__ __forceinline__ bool cycle_local_children(WaveState &W, WaveLds &L, uint32_t sinfo, int n, int sw, int T, int e,
-                                     uint32_t gmask, uint32_t amask, bool root_mode, int &scount, int &goal_t) {
+
+__device__ __forceinline__ bool cycle_local_children(WaveState &W, WaveLds &L, uint32_t sinfo, int n, int sw, int T,
+                                                     int e, uint32_t gmask, uint32_t amask, bool root_mode,
+                                                     int &scount, int &goal_t) {
     const int s = W.lane;
     const int root = (int)(sinfo & 63u), pos = (int)((sinfo >> 8) & 63u), len = (int)((sinfo >> 16) & 127u);
     const bool is_root = s < n && root == s;
-    if (__ballot(is_root && (len & 1) == 0) != 0) return false;
-    const bool big = is_root && len >= 3;
-    int kc = big ? choose3(len) : 0;
-    int lc = big ? len : 0;
+    const bool big = is_root && len >= 3;            // cycles that hold a whole triple
+    const bool ev = is_root && (len & 1) == 0;        // even cycles (length >= 2)
+    const bool listed = is_root && len >= 2;          // cycles in the member table
     // inclusive prefix sums over the lanes (cycles in root order)
-#pragma unroll
-    for (int off = 1; off < kWave; off <<= 1) {
-        const int kv = __shfl_up(kc, off), lv = __shfl_up(lc, off);
-        if (s >= off) { kc += kv; lc += lv; }
-    }
-    const int K = __shfl(kc, kWave - 1);
+    const int ka = wave_scan_add(big ? choose3(len) : 0);
+    const int mo = wave_scan_add(listed ? len : 0);
+    const int eo = wave_scan_add(ev ? len : 0);
+    const int KA = __builtin_amdgcn_readlane(ka, 63);
+    const int E = __builtin_amdgcn_readlane(eo, 63);
+    const int kb = wave_scan_add(ev ? choose2(len) * (E - len) : 0);
+    const int KB = __builtin_amdgcn_readlane(kb, 63);
+    const int K = KA + KB;
     if (K > kCycleLocalMax || 4 * K > T) return false;
-    const unsigned long long roots = __ballot(big);
-    const int kfirst = kc - (big ? choose3(len) : 0);
-    const int moff = lc - (big ? len : 0);
+    const unsigned long long roots_a = __ballot(big), roots_e = __ballot(ev);
+    const int moff = mo - (listed ? len : 0);
+    const int eoff = eo - (ev ? len : 0);
     if (big) {
-        const int c = __popcll(roots & ((1ULL << s) - 1ULL));
-        L.cyc[c] = (uint32_t)kfirst | ((uint32_t)len << 16) | ((uint32_t)moff << 24);
+        const int c = __popcll(roots_a & ((1ULL << s) - 1ULL));
+        L.cyc[c] = (uint32_t)(ka - choose3(len)) | ((uint32_t)len << 16) | ((uint32_t)moff << 24);
     }
-    const int my_off = __shfl(moff, root);
-    if (s < n && len >= 3) L.memb[my_off + pos] = (uint8_t)s;
+    if (ev) {
+        const int c = __popcll(roots_e & ((1ULL << s) - 1ULL));
+        L.ecyc[c] = (uint32_t)(kb - choose2(len) * (E - len)) | ((uint32_t)len << 16) | ((uint32_t)eoff << 24);
+    }
+    const int my_moff = __shfl(moff, root), my_eoff = __shfl(eoff, root);
+    if (s < n && len >= 2) L.memb[my_moff + pos] = (uint8_t)s;
+    if (s < n && (len & 1) == 0) L.ememb[my_eoff + pos] = (uint8_t)s;
     const uint8_t *pi = reinterpret_cast<const uint8_t *>(L.node + kNodeHeaderWords) + 4 * sw;
     if (s < n) L.posof[pi[s]] = (uint8_t)s;
     wave_sync();
-    const int ncyc = __popcll(roots);
+    const int nca = __popcll(roots_a), nce = __popcll(roots_e);
     uint32_t *tmp_t = &L.ckey[0][0];
     uint32_t *tmp_e = &L.ckey[8][0];
     int best = -1, cnt = 0;
     for (int g0 = 0; g0 < K; g0 += kWave) {
         const int g = g0 + s;
         const bool valid = g < K;
-        int c = 0;
-        for (int q = 1; q < ncyc; ++q) c = ((int)(L.cyc[q] & 0xFFFFu) <= g) ? q : c;
-        const uint32_t cw = L.cyc[c];
-        const int cl = (int)((cw >> 16) & 0xFFu), co = (int)(cw >> 24);
-        int r = valid ? g - (int)(cw & 0xFFFFu) : 0;
-        int p2 = 2;
-        while (p2 + 1 < cl && choose3(p2 + 1) <= r) ++p2;
-        r -= choose3(p2);
-        int p1 = 1;
-        while (p1 + 1 < p2 && choose2(p1 + 1) <= r) ++p1;
-        const int p0 = r - choose2(p1);
-        int a = L.posof[L.memb[co + p0]], b = L.posof[L.memb[co + p1]], x = L.posof[L.memb[co + p2]];
+        int xa, xb, xc;   // the three symbols
+        if (!valid) {
+            xa = pi[0];
+            xb = pi[1];
+            xc = pi[2];
+        } else if (g < KA) {
+            // three symbols of one cycle: colex rank within the cycle
+            const int gg = g;
+            int c = 0;
+            for (int q = 1; q < nca; ++q) c = ((int)(L.cyc[q] & 0xFFFFu) <= gg) ? q : c;
+            const uint32_t cw = L.cyc[c];
+            const int cl = (int)((cw >> 16) & 0xFFu), co = (int)(cw >> 24);
+            int r = gg - (int)(cw & 0xFFFFu);
+            int p2 = 2;
+            while (p2 + 1 < cl && choose3(p2 + 1) <= r) ++p2;
+            r -= choose3(p2);
+            int p1 = 1;
+            while (p1 + 1 < p2 && choose2(p1 + 1) <= r) ++p1;
+            const int p0 = r - choose2(p1);
+            xa = L.memb[co + p0];
+            xb = L.memb[co + p1];
+            xc = L.memb[co + p2];
+        } else {
+            // two symbols of one even cycle, the third from another even cycle
+            const int gg = g - KA;
+            int c = 0;
+            for (int q = 1; q < nce; ++q) c = ((int)(L.ecyc[q] & 0xFFFFu) <= gg) ? q : c;
+            const uint32_t cw = L.ecyc[c];
+            const int cl = (int)((cw >> 16) & 0xFFu), co = (int)(cw >> 24);
+            const int r = gg - (int)(cw & 0xFFFFu);
+            const int others = E - cl;
+            const int pair = r / others, zi = r - pair * others;
+            int p1 = 1;
+            while (p1 + 1 < cl && choose2(p1 + 1) <= pair) ++p1;
+            const int p0 = pair - choose2(p1);
+            xa = L.ememb[co + p0];
+            xb = L.ememb[co + p1];
+            xc = L.ememb[zi < co ? zi : zi + cl];
+        }
+        const int a = L.posof[xa], b = L.posof[xb], x = L.posof[xc];
         const int i = min(a, min(b, x)), k = max(a, max(b, x));
         const int j = a + b + x - i - k;
         const int delta = parity_delta(L.posinfo[i], L.posinfo[j], L.posinfo[k]);
@@ -964,8 +1022,7 @@ __device__ __forceinline__ bool cycle_local_children(WaveState &W, WaveLds &L, u
         }
         cnt += __popcll(am);
     }
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) best = max(best, __shfl_xor(best, off));
+    best = wave_max(best);
     if (best >= 0) {
         goal_t = best;
         return true;
