@@ -1372,10 +1372,13 @@ __device__ __forceinline__ void search_body(const SearchParams &P) {
 // pointers.  Three waves per SIMD (<= 168 VGPRs) costs a few spilled
 // registers; four would spill heavily.  The learned-ordering variant is a
 // kernel of its own so that the plain search carries no MFMA code.
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void search_kernel(SearchParams P) {
+#ifndef TDP_WAVES_PER_EU
+#define TDP_WAVES_PER_EU 3
+#endif
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TDP_WAVES_PER_EU))) void search_kernel(SearchParams P) {
     search_body<false>(P);
 }
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void search_learned_kernel(SearchParams P) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TDP_WAVES_PER_EU))) void search_learned_kernel(SearchParams P) {
     search_body<true>(P);
 }
 
