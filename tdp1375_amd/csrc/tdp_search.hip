@@ -50,11 +50,10 @@ enum DedupOutcome { kDedupPending = -1, kDedupDominated = 0, kDedupImproved = 1,
 struct WaveLds {
     uint32_t node[48];
     uint32_t posinfo[64];
-    uint8_t memb[64];                        // cycle members in spi order, cycles of length >= 2
-    uint8_t ememb[64];                       // the members of the even cycles again, contiguous
+    uint32_t memb[64];                       // cycle members in spi order, cycles of length >= 2:
+                                             // cycle info | position in pi << 24
+    uint32_t ememb[64];                      // the members of the even cycles again, contiguous
     uint8_t posof[64];                       // position of each symbol in pi
-    uint32_t cyc[24];                        // cycles of length >= 3: first candidate | len << 16 | memb offset << 24
-    uint32_t ecyc[32];                       // even cycles: first candidate | len << 16 | ememb offset << 24
     uint32_t stage[kStage];
     uint8_t rowsrc[64];                      // learned ordering: lane of each compacted admitted child
     uint32_t ckey[16][kWave];                // child spi words of each lane's staged child
@@ -186,6 +185,8 @@ __device__ __forceinline__ uint32_t child_spi_word(const uint32_t *pw, int q, co
 // yields any triple from its offset o; the lexicographic index is
 // t = C(n,3) - 1 - o.
 constexpr int kColexPairs = 1953;   // C(63, 2)
+// Most candidates of the cycle-local enumeration (more: the full scan runs)
+constexpr int kCycleLocalMax = 512;
 
 __device__ __forceinline__ int choose3(int m) { return m * (m - 1) * (m - 2) / 6; }
 __device__ __forceinline__ int choose2(int v) { return v * (v - 1) / 2; }
@@ -271,12 +272,23 @@ __device__ __forceinline__ int scan_children(const uint16_t *colex, int n, int T
     return kScanGo;
 }
 
-__device__ __forceinline__ void fill_colex(uint16_t *colex) {
+__device__ __forceinline__ void fill_colex(uint16_t *colex, uint32_t *colex3 = nullptr) {
     // q -> (u, v) with v the largest such that C(v,2) <= q
     for (int q = threadIdx.x; q < kColexPairs; q += blockDim.x) {
         int v = 1;
         while (choose2(v + 1) <= q) ++v;
         colex[q] = (uint16_t)((q - choose2(v)) | (v << 8));
+    }
+    // r -> (p0, p1, p2), p0 < p1 < p2, with C(p2,3) + C(p1,2) + p0 = r
+    if (colex3) {
+        for (int r = threadIdx.x; r < kCycleLocalMax; r += blockDim.x) {
+            int p2 = 2;
+            while (choose3(p2 + 1) <= r) ++p2;
+            const int r1 = r - choose3(p2);
+            int p1 = 1;
+            while (choose2(p1 + 1) <= r1) ++p1;
+            colex3[r] = (uint32_t)(r1 - choose2(p1)) | ((uint32_t)p1 << 8) | ((uint32_t)p2 << 16);
+        }
     }
     __syncthreads();
 }
@@ -334,6 +346,7 @@ __device__ __forceinline__ uint32_t build_posinfo(WaveLds &L, int n, int sw, int
 struct WaveState {
     int wave, lane;
     const uint16_t *colex;         // workgroup's colex pair table (LDS)
+    const uint32_t *colex3;        // workgroup's colex triple table (LDS)
     uint32_t lo, hi;               // private stack window [lo, hi), wraps mod 2^32
     int cur_cfg;
     bool configs_left;
@@ -619,10 +632,12 @@ __device__ __forceinline__ void push_child(const SearchParams &P, WaveState &W, 
 // inserted).  Each lane reads its bucket's eight tags as independent loads
 // (one 64-byte line, one memory round trip), compares the stored key of every
 // ready tag with its identity (epoch, configuration, 27 hash bits), and
-// settles the depth with a CAS on the tag word.  A full bucket evicts an entry
+// settles the depth from that tag: dominated, or a CAS from it that lowers the
+// depth (a failed CAS repeats the lookup).  A full bucket evicts an entry
 // of a configuration already found, else the deepest entry not shallower than
 // the child (its subtree is the cheapest to search again); a bucket of
-// shallower entries admits the child without an entry.  Losing an entry only
+// shallower entries admits the child without an entry, and no entry with the
+// child's own identity is evicted for it.  Losing an entry only
 // costs repeated work: a verdict never depends on what the table holds.
 // While no bucket fills the table is exact, as the CPU oracle's.
 struct LaneTraffic {
@@ -652,13 +667,18 @@ __device__ __forceinline__ unsigned long long key_pair(const WaveLds &L, int lan
 __device__ __forceinline__ int dedup_admit(const SearchParams &P, WaveLds &L, int lane, bool active, uint32_t cfg,
                                            int nw, int depth, LaneTraffic &tr) {
     if (P.dd_buckets == 0) return active ? kDedupUninserted : kDedupDominated;
-    uint64_t h = 0x9E3779B97F4A7C15ULL * (uint64_t)(cfg + 1);
-    for (int q = 0; q < nw; ++q) {
-        h ^= L.ckey[q][lane];
-        h *= 0xff51afd7ed558ccdULL;
-        h ^= h >> 29;
+    // two 32-bit multiplicative lanes over the even and the odd key words
+    // (each step a bijection of the lane state, so keys that differ in one
+    // word get different 64-bit hashes), joined by the MurmurHash3 64-bit
+    // finaliser
+    uint32_t h1 = 0x9E3779B9u * (cfg + 1u), h2 = 0x85EBCA6Bu ^ cfg;
+    for (int q = 0; q < nw; q += 2) {
+        h1 = (h1 ^ L.ckey[q][lane]) * 0xCC9E2D51u;
+        h1 ^= h1 >> 15;
+        h2 = (h2 ^ (q + 1 < nw ? L.ckey[q + 1][lane] : 0u)) * 0x1B873593u;
+        h2 ^= h2 >> 13;
     }
-    h = fmix64(h);
+    const uint64_t h = fmix64(((uint64_t)h1 << 32) | h2);
     const unsigned long long ident = dedup_ident(P.epoch, cfg, h);
     const unsigned long long bucket = __umul64hi(h, P.dd_buckets);
     unsigned long long *tags = P.dd_tag + bucket * kDedupBucket;
@@ -681,6 +701,7 @@ __device__ __forceinline__ int dedup_admit(const SearchParams &P, WaveLds &L, in
                 busy |= same && !(t[q] & kTagReady);
                 cand |= (same && (t[q] & kTagReady)) ? (1u << q) : 0u;
             }
+            const uint32_t same_ident = cand;   // never evicted for this child
             // 1. compare the stored keys of the candidates
 #pragma unroll 1
             while (cand && res == kDedupPending && !retry) {
@@ -700,15 +721,15 @@ __device__ __forceinline__ int dedup_admit(const SearchParams &P, WaveLds &L, in
                     if (w < nw2) eq &= kv[w] == key_pair(L, lane, w, nw);
                 tr.key_reads++;
                 if (!eq) continue;
-                // the key read belongs to this entry only if the tag did not
-                // change meanwhile (an eviction rewrites the identity first)
-                complete_after();
-                const unsigned long long cu = ld_relaxed(tags + q);
-                if ((cu & ~15ULL) != (tq & ~15ULL)) {
-                    retry = true;
-                } else if ((int)(cu & 15ULL) <= depth) {
+                // The snapshot tag decides: it was an admission of this key at
+                // its depth.  No entry is replaced by one of the same identity
+                // (victim choice below), so a key rewritten by an eviction
+                // meanwhile has another identity and stands in for the
+                // snapshot's only under a 27-bit hash collision, the odds of
+                // an ABA on the tag word.
+                if ((int)(tq & 15ULL) <= depth) {
                     res = kDedupDominated;
-                } else if (cas64(tags + q, cu, (cu & ~15ULL) | (unsigned long long)depth)) {
+                } else if (cas64(tags + q, tq, (tq & ~15ULL) | (unsigned long long)depth)) {
                     res = kDedupImproved;
                 } else {
                     retry = true;
@@ -729,7 +750,7 @@ __device__ __forceinline__ int dedup_admit(const SearchParams &P, WaveLds &L, in
                         uint32_t found_mask = 0;
 #pragma unroll
                         for (int q = 0; q < kDedupBucket; ++q) {
-                            if (q < width && (t[q] & kTagReady)) {
+                            if (q < width && (t[q] & kTagReady) && !((same_ident >> q) & 1u)) {
                                 const uint32_t f = (uint32_t)__hip_atomic_load(
                                     &P.results[(t[q] >> 32) & 0xFFFFu].found, __ATOMIC_RELAXED,
                                     __HIP_MEMORY_SCOPE_AGENT);
@@ -744,7 +765,10 @@ __device__ __forceinline__ int dedup_admit(const SearchParams &P, WaveLds &L, in
 #pragma unroll
                             for (int q = 0; q < kDedupBucket; ++q) {
                                 const int dq = (int)(t[q] & 15ULL);
-                                if (q < width && (t[q] & kTagReady) && dq > vd) { vd = dq; victim = q; }
+                                if (q < width && (t[q] & kTagReady) && !((same_ident >> q) & 1u) && dq > vd) {
+                                    vd = dq;
+                                    victim = q;
+                                }
                             }
                             live = true;
                         }
@@ -920,7 +944,6 @@ __device__ __forceinline__ void report_found(const SearchParams &P, WaveState &W
 // triple order.  Returns false (nothing done) when the candidates are too
 // many for the stage or not clearly fewer than C(n,3); otherwise sets
 // scount / goal_t as the scan would.
-constexpr int kCycleLocalMax = 512;
 
 
 __device__ __forceinline__ bool cycle_local_children(WaveState &W, WaveLds &L, uint32_t sinfo, int n, int sw, int T,
@@ -945,96 +968,105 @@ __device__ __forceinline__ bool cycle_local_children(WaveState &W, WaveLds &L, u
     const unsigned long long roots_a = __ballot(big), roots_e = __ballot(ev);
     const int moff = mo - (listed ? len : 0);
     const int eoff = eo - (ev ? len : 0);
-    if (big) {
-        const int c = __popcll(roots_a & ((1ULL << s) - 1ULL));
-        L.cyc[c] = (uint32_t)(ka - choose3(len)) | ((uint32_t)len << 16) | ((uint32_t)moff << 24);
-    }
-    if (ev) {
-        const int c = __popcll(roots_e & ((1ULL << s) - 1ULL));
-        L.ecyc[c] = (uint32_t)(kb - choose2(len) * (E - len)) | ((uint32_t)len << 16) | ((uint32_t)eoff << 24);
-    }
-    const int my_moff = __shfl(moff, root), my_eoff = __shfl(eoff, root);
-    if (s < n && len >= 2) L.memb[my_moff + pos] = (uint8_t)s;
-    if (s < n && (len & 1) == 0) L.ememb[my_eoff + pos] = (uint8_t)s;
+    // per cycle, in its root lane: first candidate | length << 16 | member
+    // offset << 24 (candidates of a cycle are consecutive, cycles in root order)
+    const uint32_t acw = (uint32_t)(ka - (big ? choose3(len) : 0)) | ((uint32_t)len << 16) | ((uint32_t)moff << 24);
+    const uint32_t ecw = (uint32_t)(kb - (ev ? choose2(len) * (E - len) : 0)) | ((uint32_t)len << 16) |
+                         ((uint32_t)eoff << 24);
+    // member entries: the symbol's cycle info with its position in pi in
+    // bits 24..29, so that one LDS read yields both and sorting three entries
+    // sorts them by position
     const uint8_t *pi = reinterpret_cast<const uint8_t *>(L.node + kNodeHeaderWords) + 4 * sw;
     if (s < n) L.posof[pi[s]] = (uint8_t)s;
+    const int my_moff = __shfl(moff, root), my_eoff = __shfl(eoff, root);
     wave_sync();
-    const int nca = __popcll(roots_a), nce = __popcll(roots_e);
-    uint32_t *tmp_t = &L.ckey[0][0];
+    const uint32_t entry = sinfo | ((uint32_t)L.posof[s < n ? s : 0] << 24);
+    if (s < n && len >= 2) L.memb[my_moff + pos] = entry;
+    if (s < n && (len & 1) == 0) L.ememb[my_eoff + pos] = entry;
+    wave_sync();
+    uint32_t *tmp_k = &L.ckey[0][0];
     uint32_t *tmp_e = &L.ckey[8][0];
     int best = -1, cnt = 0;
     for (int g0 = 0; g0 < K; g0 += kWave) {
         const int g = g0 + s;
         const bool valid = g < K;
-        int xa, xb, xc;   // the three symbols
+        uint32_t ma, mb, mc;
         if (!valid) {
-            xa = pi[0];
-            xb = pi[1];
-            xc = pi[2];
+            // three distinct positions, result masked below
+            ma = L.posinfo[0];
+            mb = L.posinfo[1] | (1u << 24);
+            mc = L.posinfo[2] | (2u << 24);
         } else if (g < KA) {
-            // three symbols of one cycle: colex rank within the cycle
-            const int gg = g;
-            int c = 0;
-            for (int q = 1; q < nca; ++q) c = ((int)(L.cyc[q] & 0xFFFFu) <= gg) ? q : c;
-            const uint32_t cw = L.cyc[c];
-            const int cl = (int)((cw >> 16) & 0xFFu), co = (int)(cw >> 24);
-            int r = gg - (int)(cw & 0xFFFFu);
-            int p2 = 2;
-            while (p2 + 1 < cl && choose3(p2 + 1) <= r) ++p2;
-            r -= choose3(p2);
-            int p1 = 1;
-            while (p1 + 1 < p2 && choose2(p1 + 1) <= r) ++p1;
-            const int p0 = r - choose2(p1);
-            xa = L.memb[co + p0];
-            xb = L.memb[co + p1];
-            xc = L.memb[co + p2];
+            // three members of one cycle: the cycle holding candidate g (the
+            // last one starting at or before it), then its colex triple
+            uint32_t cw = 0;
+            for (unsigned long long rm = roots_a; rm; rm &= rm - 1) {
+                const uint32_t w = (uint32_t)__builtin_amdgcn_readlane((int)acw, __ffsll((long long)rm) - 1);
+                cw = ((int)(w & 0xFFFFu) <= g) ? w : cw;
+            }
+            const uint32_t tri = W.colex3[g - (int)(cw & 0xFFFFu)];
+            const int co = (int)(cw >> 24);
+            ma = L.memb[co + (int)(tri & 0xFFu)];
+            mb = L.memb[co + (int)((tri >> 8) & 0xFFu)];
+            mc = L.memb[co + (int)(tri >> 16)];
         } else {
-            // two symbols of one even cycle, the third from another even cycle
+            // two members of one even cycle, the third from another even cycle
             const int gg = g - KA;
-            int c = 0;
-            for (int q = 1; q < nce; ++q) c = ((int)(L.ecyc[q] & 0xFFFFu) <= gg) ? q : c;
-            const uint32_t cw = L.ecyc[c];
+            uint32_t cw = 0;
+            for (unsigned long long rm = roots_e; rm; rm &= rm - 1) {
+                const uint32_t w = (uint32_t)__builtin_amdgcn_readlane((int)ecw, __ffsll((long long)rm) - 1);
+                cw = ((int)(w & 0xFFFFu) <= gg) ? w : cw;
+            }
             const int cl = (int)((cw >> 16) & 0xFFu), co = (int)(cw >> 24);
             const int r = gg - (int)(cw & 0xFFFFu);
             const int others = E - cl;
             const int pair = r / others, zi = r - pair * others;
-            int p1 = 1;
-            while (p1 + 1 < cl && choose2(p1 + 1) <= pair) ++p1;
-            const int p0 = pair - choose2(p1);
-            xa = L.ememb[co + p0];
-            xb = L.ememb[co + p1];
-            xc = L.ememb[zi < co ? zi : zi + cl];
+            const uint32_t pw = W.colex[pair];
+            ma = L.ememb[co + (int)(pw & 0xFFu)];
+            mb = L.ememb[co + (int)(pw >> 8)];
+            mc = L.ememb[zi < co ? zi : zi + cl];
         }
-        const int a = L.posof[xa], b = L.posof[xb], x = L.posof[xc];
-        const int i = min(a, min(b, x)), k = max(a, max(b, x));
-        const int j = a + b + x - i - k;
-        const int delta = parity_delta(L.posinfo[i], L.posinfo[j], L.posinfo[k]);
+        // sorted by position (distinct positions in bits 24..29)
+        const uint32_t lo = min(ma, min(mb, mc)), hi = max(ma, max(mb, mc));
+        const uint32_t mid = ma ^ mb ^ mc ^ lo ^ hi;
+        const int i = (int)(lo >> 24), j = (int)(mid >> 24), k = (int)(hi >> 24);
+        const int delta = parity_delta(lo, mid, hi);
         const uint32_t di = (uint32_t)(delta + 2) >> 1;
         const bool goal = valid && !root_mode && ((gmask >> di) & 1u);
         const bool adm = valid && ((amask >> di) & 1u);
-        const int t = lex_index(n, T, i, j, k);
-        if (goal) best = max(best, t);
+        // i << 12 | j << 6 | k grows with the lexicographic triple index
+        const int key = (i << 12) | (j << 6) | k;
+        if (goal) best = max(best, key);
         const unsigned long long am = __ballot(adm);
         const unsigned long long below = (s == 0) ? 0ULL : (am & (~0ULL >> (64 - s)));
         if (adm) {
-            tmp_t[cnt + __popcll(below)] = (uint32_t)t;
+            tmp_k[cnt + __popcll(below)] = (uint32_t)key;
             tmp_e[cnt + __popcll(below)] = stage_entry(i, j, k, e + delta);
         }
         cnt += __popcll(am);
     }
     best = wave_max(best);
     if (best >= 0) {
-        goal_t = best;
+        goal_t = lex_index(n, T, best >> 12, (best >> 6) & 63, best & 63);
         return true;
     }
     wave_sync();
     // descending triple order: the rank of an entry is the number of larger
-    // triple indices (all distinct)
-    for (int idx = s; idx < cnt; idx += kWave) {
-        const uint32_t tt = tmp_t[idx];
+    // keys (all distinct); up to 64 entries rank against registers
+    if (cnt <= kWave) {
+        const uint32_t mk = s < cnt ? tmp_k[s] : 0u;
+        const uint32_t me = s < cnt ? tmp_e[s] : 0u;
         int rank = 0;
-        for (int q = 0; q < cnt; ++q) rank += tmp_t[q] > tt ? 1 : 0;
-        L.stage[rank] = tmp_e[idx];
+        for (int q = 0; q < cnt; ++q) rank += (uint32_t)__builtin_amdgcn_readlane((int)mk, q) > mk ? 1 : 0;
+        wave_sync();
+        if (s < cnt) L.stage[rank] = me;
+    } else {
+        for (int idx = s; idx < cnt; idx += kWave) {
+            const uint32_t kk = tmp_k[idx];
+            int rank = 0;
+            for (int q = 0; q < cnt; ++q) rank += tmp_k[q] > kk ? 1 : 0;
+            L.stage[rank] = tmp_e[idx];
+        }
     }
     if (root_mode) lds_add(&L.cnt[kCntRootsLive], (unsigned long long)cnt);
     wave_sync();
@@ -1174,7 +1206,8 @@ template <bool kLearned>
 __device__ __forceinline__ void search_body(const SearchParams &P) {
     __shared__ WaveLds lds[kWavesPerBlock];
     __shared__ uint16_t colex[kColexPairs];
-    fill_colex(colex);
+    __shared__ uint32_t colex3[kCycleLocalMax];
+    fill_colex(colex, colex3);
     const int wib = threadIdx.x >> 6;
     WaveLds &L = lds[wib];
     WaveState W;
@@ -1182,6 +1215,7 @@ __device__ __forceinline__ void search_body(const SearchParams &P) {
     W.wave = blockIdx.x * kWavesPerBlock + wib;
     if (W.wave >= P.num_waves) return;
     W.colex = colex;
+    W.colex3 = colex3;
     W.lo = W.hi = 0;
     W.cur_cfg = -1;
     W.configs_left = true;
