@@ -178,6 +178,12 @@ TDP_API int tdp_host_predicate_tables(int n, int initial_even_cycles, float min_
 TDP_API int tdp_set_tuning(int waves_per_cu, int share_interval, int share_max, int idle_backoff,
                            int max_active);
 
+/* Nodes a wave of the work-sharing mode takes from its stack and expands
+ * before one dedup pass over all their children (1..4; 0 = default 4; the
+ * deterministic modes and the learned ordering always take one).  Extension;
+ * no reference counterpart. */
+TDP_API int tdp_set_expansion_batch(int nodes);
+
 /* ---- learned move ordering (extension; no reference counterpart) ----
  * A 128 -> 256 (ReLU) -> 1 MLP scores the candidate moves of a node from 128
  * feature bits (tdp_learn.h); searches with TDP_FLAG_LEARNED push the admitted

@@ -8,10 +8,10 @@ from ._lib import (FLAG_DETERMINISTIC, FLAG_SINGLE_WAVE, CancelledError, QueueOv
                    load)
 from .gpu_sorting_search import (GPUSortingSearch, cancel_sort, destroy_slots, filter_pi_permutations,
                                  init_slots, last_launch_stats, positions_to_symbols, search_batch,
-                                 search_for_sorting_seq, set_tuning)
+                                 search_for_sorting_seq, set_expansion_batch, set_tuning)
 
 __all__ = [
     "GPUSortingSearch", "init_slots", "destroy_slots", "cancel_sort", "filter_pi_permutations",
-    "search_for_sorting_seq", "search_batch", "set_tuning", "positions_to_symbols", "last_launch_stats", "load",
+    "search_for_sorting_seq", "search_batch", "set_tuning", "set_expansion_batch", "positions_to_symbols", "last_launch_stats", "load",
     "TdpError", "CancelledError", "QueueOverflowError", "FLAG_DETERMINISTIC", "FLAG_SINGLE_WAVE",
 ]

@@ -409,6 +409,7 @@ struct Tuning {
     int idle_backoff = 16;    // longest idle sleep, x 127 s_sleep units
     int max_active = 512;     // configurations per persistent launch
     int deadline_ms = 1000;   // TDP_FLAG_SOFT_DEADLINE: time budget of each launch
+    int batch = 4;            // nodes a wave expands before one dedup pass (plain ordering)
 };
 std::mutex g_tuning_mutex;
 Tuning g_tuning;
@@ -702,6 +703,9 @@ void run_launch(Slot &s, const std::vector<CaseInput> &cases, uint32_t flags, bo
     P.cancel_flag = as<int>(s.cancel_flag);
     P.donate = donate ? 1 : 0;
     P.share_mask = (uint32_t)tune.share_interval - 1;
+    // the fixed expansion order of the deterministic modes and the learned
+    // ordering (which scores one node's children) take one node at a time
+    P.batch = (donate && !(flags & TDP_FLAG_LEARNED)) ? tune.batch : 1;
     P.share_max = share_max;
     P.idle_backoff = (uint32_t)tune.idle_backoff;
     P.timeout_ticks = soft ? (unsigned long long)tune.deadline_ms * 100000ULL : kernel_timeout_ticks();
@@ -1026,6 +1030,14 @@ int tdp_set_launch_deadline(int milliseconds) {
     });
 }
 
+int tdp_set_expansion_batch(int nodes) {
+    return guarded([&] {
+        if (nodes < 0 || nodes > kExpansionBatchMax) throw std::invalid_argument("batch must be in 0..4");
+        std::lock_guard<std::mutex> lk(g_tuning_mutex);
+        g_tuning.batch = nodes ? nodes : Tuning().batch;
+    });
+}
+
 int tdp_set_tuning(int waves_per_cu, int share_interval, int share_max, int idle_backoff, int max_active) {
     return guarded([&] {
         if (waves_per_cu < 0 || share_interval < 0 || share_max < 0 || idle_backoff < 0 || max_active < 0)
@@ -1034,6 +1046,7 @@ int tdp_set_tuning(int waves_per_cu, int share_interval, int share_max, int idle
             throw std::invalid_argument("share_interval must be a power of two");
         Tuning t;
         t.deadline_ms = current_tuning().deadline_ms;
+        t.batch = current_tuning().batch;
         if (waves_per_cu) t.waves_per_cu = waves_per_cu;
         if (share_interval) t.share_interval = share_interval;
         if (share_max) t.share_max = share_max;

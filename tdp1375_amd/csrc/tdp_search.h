@@ -9,6 +9,9 @@
 
 namespace tdp {
 
+// Most nodes a wave expands before one dedup pass over all their children.
+constexpr int kExpansionBatchMax = 4;
+
 struct SearchParams {
     const ConfigDesc *cfgs;
     ConfigResult *results;
@@ -46,6 +49,7 @@ struct SearchParams {
     const int *cancel_flag;
     int donate;                   // 0: every configuration stays on one wave
     uint32_t share_mask;          // donation check every share_mask + 1 expansions
+    int batch;                    // nodes expanded before one dedup pass, 1..kExpansionBatchMax
     uint32_t share_max;           // nodes handed over per donation
     uint32_t idle_backoff;        // longest idle sleep, in units of s_sleep 127
     unsigned long long timeout_ticks;  // watchdog: 100 MHz ticks per wave, 0 = none

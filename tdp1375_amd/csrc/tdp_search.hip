@@ -38,6 +38,7 @@ namespace {
 
 constexpr int kWavesPerBlock = 4;
 constexpr int kStage = 512;
+constexpr int kBatchMax = kExpansionBatchMax;
 // Rounds of one dedup lookup before the child is admitted without an entry
 // (a round is lost only to a concurrent update of the same bucket).
 constexpr int kMaxDedupRounds = 16;
@@ -48,7 +49,7 @@ enum DedupOutcome { kDedupPending = -1, kDedupDominated = 0, kDedupImproved = 1,
 // Per-wave LDS slice.  The 64-bit counters live here rather than in
 // registers (lane 0 updates them with wave-uniform values).
 struct WaveLds {
-    uint32_t node[48];
+    uint32_t node[kBatchMax][48];            // the nodes being expanded (node[0] alone outside batches)
     uint32_t posinfo[64];
     uint32_t memb[64];                       // cycle members in spi order, cycles of length >= 2:
                                              // cycle info | position in pi << 24
@@ -65,8 +66,10 @@ struct WaveLds {
     unsigned long long phase[kNumPhases];    // 10 ns ticks per phase (TDP_PHASE_TIMING builds)
 };
 
+// An LDS atomic add whose result is unused (ds_add_u64 without return), so
+// the wave does not wait for the read of a read-modify-write.
 __device__ __forceinline__ void lds_add(unsigned long long *p, unsigned long long v) {
-    if ((threadIdx.x & 63) == 0) *p += v;
+    if ((threadIdx.x & 63) == 0) __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
 }
 
 // Optional per-phase wall time (s_memrealtime, 100 MHz) for profiling builds
@@ -293,9 +296,9 @@ __device__ __forceinline__ void fill_colex(uint16_t *colex, uint32_t *colex3 = n
     __syncthreads();
 }
 
-// Stage entries: i | j << 6 | k << 12 | even << 18.
-__device__ __forceinline__ uint32_t stage_entry(int i, int j, int k, int even) {
-    return (uint32_t)i | ((uint32_t)j << 6) | ((uint32_t)k << 12) | ((uint32_t)even << 18);
+// Stage entries: i | j << 6 | k << 12 | even << 18 | parent node << 24.
+__device__ __forceinline__ uint32_t stage_entry(int i, int j, int k, int even, int b) {
+    return (uint32_t)i | ((uint32_t)j << 6) | ((uint32_t)k << 12) | ((uint32_t)even << 18) | ((uint32_t)b << 24);
 }
 
 // transposition.cuh:5-11
@@ -308,8 +311,8 @@ __device__ __forceinline__ int transposed_src_index(int p, int i, int j, int k) 
 
 // Lane s handles symbol s: cycle id (min symbol) and distance to it by
 // pointer jumping over ds_bpermute, then posinfo[p] = info(pi[p]).
-__device__ __forceinline__ uint32_t build_posinfo(WaveLds &L, int n, int sw, int lane) {
-    const uint8_t *spi = reinterpret_cast<const uint8_t *>(L.node + kNodeHeaderWords);
+__device__ __forceinline__ uint32_t build_posinfo(WaveLds &L, const uint32_t *nd, int n, int sw, int lane) {
+    const uint8_t *spi = reinterpret_cast<const uint8_t *>(nd + kNodeHeaderWords);
     const uint8_t *pi = spi + 4 * sw;
     const int s = lane;
     const int nxt = (s < n) ? (int)spi[s] : s;
@@ -548,7 +551,7 @@ __device__ __forceinline__ bool take_one(const SearchParams &P, WaveState &W, Wa
     if (__shfl(late, 0)) { set_watchdog(P, W); return false; }
     complete_after();
     const uint32_t *src = P.pool_nodes + slot * P.node_words;
-    if (W.lane < (int)P.node_words) L.node[W.lane] = ld32(src + W.lane);
+    if (W.lane < (int)P.node_words) L.node[0][W.lane] = ld32(src + W.lane);
     complete_before();
     if (W.lane == 0) {
         st64(&P.pool_seq[slot], ticket + P.pool_cap);
@@ -586,16 +589,12 @@ __device__ __forceinline__ bool make_room(const SearchParams &P, WaveState &W) {
 }
 
 // Pushes the child (t, i, j, k, even) of the node in L.node.
-__device__ __forceinline__ void push_child(const SearchParams &P, WaveState &W, WaveLds &L, int n, int sw,
-                           int t, int i, int j, int k, int even) {
+__device__ __forceinline__ void push_child(const SearchParams &P, WaveState &W, WaveLds &L, const uint32_t *pw,
+                                           int n, int sw, int t, int i, int j, int k, int even, const ChildMove &m) {
     if (!make_room(P, W)) return;
-    const uint32_t *pw = L.node;
     const uint8_t *spi = reinterpret_cast<const uint8_t *>(pw + kNodeHeaderWords);
     const uint8_t *pi = spi + 4 * sw;
     const int d = (int)(pw[1] & 0xFF);
-    ChildMove m;
-    m.a = pi[i]; m.b = pi[j]; m.c = pi[k];
-    m.na = spi[m.c]; m.nb = spi[m.a]; m.nc = spi[m.b];
     uint32_t *dst = stack_slot(P, W, W.hi);
     const int lane = W.lane;
     if (lane < (int)P.node_words) {
@@ -640,11 +639,15 @@ __device__ __forceinline__ void push_child(const SearchParams &P, WaveState &W, 
 // child's own identity is evicted for it.  Losing an entry only
 // costs repeated work: a verdict never depends on what the table holds.
 // While no bucket fills the table is exact, as the CPU oracle's.
-struct LaneTraffic {
-    int probes, key_reads, key_writes, evictions;
-};
+// Counts of tag reads, key compares, key writes and evictions go to
+// L.traffic[2..5] as popcounts of the lanes taking part (first active lane
+// adds, no wait).
+__device__ __forceinline__ void count_active(unsigned long long *p, unsigned long long lanes) {
+    const unsigned long long ex = __builtin_amdgcn_read_exec();
+    if ((int)(threadIdx.x & 63) == __ffsll((long long)ex) - 1)
+        __hip_atomic_fetch_add(p, lanes, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+}
 
-__device__ __forceinline__ int wave_sum(int v) { return __builtin_amdgcn_readlane(wave_scan_add(v), 63); }
 
 // t[q] for a wave-varying q without dynamic register indexing (which would
 // put the array in scratch memory)
@@ -665,7 +668,7 @@ __device__ __forceinline__ unsigned long long key_pair(const WaveLds &L, int lan
 }
 
 __device__ __forceinline__ int dedup_admit(const SearchParams &P, WaveLds &L, int lane, bool active, uint32_t cfg,
-                                           int nw, int depth, LaneTraffic &tr) {
+                                           int nw, int depth) {
     if (P.dd_buckets == 0) return active ? kDedupUninserted : kDedupDominated;
     // two 32-bit multiplicative lanes over the even and the odd key words
     // (each step a bijection of the lane state, so keys that differ in one
@@ -687,12 +690,14 @@ __device__ __forceinline__ int dedup_admit(const SearchParams &P, WaveLds &L, in
     const unsigned long long epoch = P.epoch;
     int res = active ? kDedupPending : kDedupDominated;
     int rounds = 0;
-    while (__ballot(res == kDedupPending) != 0) {
+    while (true) {
+        const unsigned long long pm = __ballot(res == kDedupPending);
+        if (pm == 0) break;
+        lds_add(&L.traffic[2], (unsigned long long)width * (unsigned long long)__popcll(pm));
         if (res == kDedupPending) {
             unsigned long long t[kDedupBucket];
 #pragma unroll
             for (int q = 0; q < kDedupBucket; ++q) t[q] = q < width ? ld_relaxed(tags + q) : 0ULL;
-            tr.probes += width;
             bool retry = false, busy = false;
             uint32_t cand = 0;   // ready entries with this identity
 #pragma unroll
@@ -719,7 +724,7 @@ __device__ __forceinline__ int dedup_admit(const SearchParams &P, WaveLds &L, in
 #pragma unroll
                 for (int w = 0; w < 8; ++w)
                     if (w < nw2) eq &= kv[w] == key_pair(L, lane, w, nw);
-                tr.key_reads++;
+                count_active(&L.traffic[3], __popcll(__builtin_amdgcn_read_exec()));
                 if (!eq) continue;
                 // The snapshot tag decides: it was an admission of this key at
                 // its depth.  No entry is replaced by one of the same identity
@@ -784,8 +789,8 @@ __device__ __forceinline__ int dedup_admit(const SearchParams &P, WaveLds &L, in
                         complete_before();
                         st64(tags + victim, ident | kTagReady | (unsigned long long)depth);
                         res = kDedupInserted;
-                        tr.key_writes++;
-                        if (live) tr.evictions++;
+                        count_active(&L.traffic[4], __popcll(__builtin_amdgcn_read_exec()));
+                        count_active(&L.traffic[5], __popcll(__ballot(live)));
                     } else {
                         retry = true;
                     }
@@ -802,29 +807,27 @@ __device__ __forceinline__ int dedup_admit(const SearchParams &P, WaveLds &L, in
 template <bool kLearned>
 __device__ __forceinline__ void flush_stage(const SearchParams &P, WaveState &W, WaveLds &L, int count, int n, int sw,
                             int T, bool root_stage) {
-    const uint32_t cfg = L.node[0];
-    const int d = (int)(L.node[1] & 0xFF);
-    const uint8_t *spi = reinterpret_cast<const uint8_t *>(L.node + kNodeHeaderWords);
-    const uint8_t *pi = spi + 4 * sw;
+    const uint32_t cfg = L.node[0][0];   // every node of a batch has the same configuration
     for (int base = 0; base < count && !W.stop; base += kWave) {
         const int idx = base + W.lane;
         const bool act = idx < count;
-        const uint32_t st = act ? L.stage[idx] : stage_entry(0, 1, 2, 0);
+        const uint32_t st = act ? L.stage[idx] : stage_entry(0, 1, 2, 0, 0);
         const int i = st & 63, j = (st >> 6) & 63, k = (st >> 12) & 63;
-        const int ce = (int)(st >> 18);
+        const int ce = (int)((st >> 18) & 63u);
+        const uint32_t *nd = L.node[st >> 24];   // the child's parent
+        const int d = (int)(nd[1] & 0xFF);
+        const uint8_t *spi = reinterpret_cast<const uint8_t *>(nd + kNodeHeaderWords);
+        const uint8_t *pi = spi + 4 * sw;
         const int t = lex_index(n, T, i, j, k);
         ChildMove m;
         m.a = pi[i]; m.b = pi[j]; m.c = pi[k];
         m.na = spi[m.c]; m.nb = spi[m.a]; m.nc = spi[m.b];
-        LaneTraffic tr{0, 0, 0, 0};
+        const uint32_t mv1 = (uint32_t)m.a | ((uint32_t)m.b << 8) | ((uint32_t)m.c << 16);
+        const uint32_t mv2 = m.na | (m.nb << 8) | (m.nc << 16);
         PHASE_T0();
-        for (int q = 0; q < sw; ++q) L.ckey[q][W.lane] = child_spi_word(L.node + kNodeHeaderWords, q, m);
-        const int r = dedup_admit(P, L, W.lane, act, cfg, sw, d + 1, tr);
+        for (int q = 0; q < sw; ++q) L.ckey[q][W.lane] = child_spi_word(nd + kNodeHeaderWords, q, m);
+        const int r = dedup_admit(P, L, W.lane, act, cfg, sw, d + 1);
         PHASE_ADD(L, kPhaseDedup);
-        lds_add(&L.traffic[2], (unsigned long long)wave_sum(tr.probes));
-        lds_add(&L.traffic[3], (unsigned long long)wave_sum(tr.key_reads));
-        lds_add(&L.traffic[4], (unsigned long long)wave_sum(tr.key_writes));
-        lds_add(&L.traffic[5], (unsigned long long)wave_sum(tr.evictions));
         const bool admit = act && r != kDedupDominated;
         if (!root_stage) {
             const unsigned long long dom = __popcll(__ballot(act && r == kDedupDominated));
@@ -845,7 +848,7 @@ __device__ __forceinline__ void flush_stage(const SearchParams &P, WaveState &W,
             // order)
             PHASE_T0();
             const ConfigDesc &C = P.cfgs[cfg];
-            const int e = (int)((L.node[1] >> 8) & 0xFF);
+            const int e = (int)((nd[1] >> 8) & 0xFF);
             uint32_t f[kFeatWords];
             move_features(L.posinfo[i], L.posinfo[j], L.posinfo[k], i, j, k, n, d, (int)C.max_depth,
                           (int)C.e_start, ce, ce - e, f);
@@ -897,9 +900,16 @@ __device__ __forceinline__ void flush_stage(const SearchParams &P, WaveState &W,
                     src = __ffsll((long long)pm) - 1;
                     pm &= pm - 1;
                 }
-                const int ct = __shfl(t, src), ci = __shfl(i, src), cj = __shfl(j, src), ck = __shfl(k, src);
-                const int cce = __shfl(ce, src);
-                push_child(P, W, L, n, sw, uniform(ct), uniform(ci), uniform(cj), uniform(ck), uniform(cce));
+                // the child's move from its lane's registers (no LDS reads)
+                const uint32_t cst = (uint32_t)__builtin_amdgcn_readlane((int)st, src);
+                const int ct = __builtin_amdgcn_readlane(t, src);
+                const uint32_t c1 = (uint32_t)__builtin_amdgcn_readlane((int)mv1, src);
+                const uint32_t c2 = (uint32_t)__builtin_amdgcn_readlane((int)mv2, src);
+                ChildMove cm;
+                cm.a = (int)(c1 & 0xFFu); cm.b = (int)((c1 >> 8) & 0xFFu); cm.c = (int)(c1 >> 16);
+                cm.na = c2 & 0xFFu; cm.nb = (c2 >> 8) & 0xFFu; cm.nc = c2 >> 16;
+                push_child(P, W, L, L.node[cst >> 24], n, sw, ct, (int)(cst & 63u), (int)((cst >> 6) & 63u),
+                           (int)((cst >> 12) & 63u), (int)((cst >> 18) & 63u), cm);
                 if (W.stop) break;
             }
             PHASE_ADD(L, kPhasePush);
@@ -907,8 +917,8 @@ __device__ __forceinline__ void flush_stage(const SearchParams &P, WaveState &W,
     }
 }
 
-__device__ __forceinline__ void report_found(const SearchParams &P, WaveState &W, WaveLds &L, int cfg, int depth,
-                             int extra_t) {
+__device__ __forceinline__ void report_found(const SearchParams &P, WaveState &W, const uint32_t *nd, int cfg,
+                                             int depth, int extra_t) {
     int won = 0;
     if (W.lane == 0) {
         int expected = 0;
@@ -921,9 +931,9 @@ __device__ __forceinline__ void report_found(const SearchParams &P, WaveState &W
     }
     won = __shfl(won, 0);
     if (won) {
-        const int pd = (int)(L.node[1] & 0xFF);
+        const int pd = (int)(nd[1] & 0xFF);
         if (W.lane < pd) {
-            const uint32_t w = L.node[2 + (W.lane >> 1)];
+            const uint32_t w = nd[2 + (W.lane >> 1)];
             P.results[cfg].moves[W.lane] = (W.lane & 1) ? (w >> 16) : (w & 0xFFFF);
         } else if (W.lane == pd && extra_t >= 0) {
             P.results[cfg].moves[W.lane] = (uint32_t)extra_t;
@@ -940,15 +950,17 @@ __device__ __forceinline__ void report_found(const SearchParams &P, WaveState &W
 //   sum over cycles C(L,3)  +  sum over even cycles C(L,2) (E - L)
 // triples are evaluated (E: symbols in even cycles), with the same parity
 // delta as the full scan, and the result is the full scan's: the goal is the
-// largest goal triple index, admitted children are staged in descending
-// triple order.  Returns false (nothing done) when the candidates are too
-// many for the stage or not clearly fewer than C(n,3); otherwise sets
-// scount / goal_t as the scan would.
+// largest goal triple index, admitted children are appended to the stage in
+// descending triple order (entries tagged with node index b).  Returns
+// kLocalNo (nothing done) when the candidates are too many or not clearly
+// fewer than C(n,3), kLocalFull when they might not fit behind the scount
+// entries already staged; otherwise advances scount / sets goal_t as the scan
+// would and returns kLocalDone.
+enum { kLocalNo = 0, kLocalDone = 1, kLocalFull = 2 };
 
-
-__device__ __forceinline__ bool cycle_local_children(WaveState &W, WaveLds &L, uint32_t sinfo, int n, int sw, int T,
-                                                     int e, uint32_t gmask, uint32_t amask, bool root_mode,
-                                                     int &scount, int &goal_t) {
+__device__ __forceinline__ int cycle_local_children(WaveState &W, WaveLds &L, const uint32_t *nd, int b,
+                                                    uint32_t sinfo, int n, int sw, int T, int e, uint32_t gmask,
+                                                    uint32_t amask, bool root_mode, int &scount, int &goal_t) {
     const int s = W.lane;
     const int root = (int)(sinfo & 63u), pos = (int)((sinfo >> 8) & 63u), len = (int)((sinfo >> 16) & 127u);
     const bool is_root = s < n && root == s;
@@ -964,7 +976,8 @@ __device__ __forceinline__ bool cycle_local_children(WaveState &W, WaveLds &L, u
     const int kb = wave_scan_add(ev ? choose2(len) * (E - len) : 0);
     const int KB = __builtin_amdgcn_readlane(kb, 63);
     const int K = KA + KB;
-    if (K > kCycleLocalMax || 4 * K > T) return false;
+    if (K > kCycleLocalMax || 4 * K > T) return kLocalNo;
+    if (scount + K > kStage) return kLocalFull;   // the caller flushes the stage first
     const unsigned long long roots_a = __ballot(big), roots_e = __ballot(ev);
     const int moff = mo - (listed ? len : 0);
     const int eoff = eo - (ev ? len : 0);
@@ -976,7 +989,7 @@ __device__ __forceinline__ bool cycle_local_children(WaveState &W, WaveLds &L, u
     // member entries: the symbol's cycle info with its position in pi in
     // bits 24..29, so that one LDS read yields both and sorting three entries
     // sorts them by position
-    const uint8_t *pi = reinterpret_cast<const uint8_t *>(L.node + kNodeHeaderWords) + 4 * sw;
+    const uint8_t *pi = reinterpret_cast<const uint8_t *>(nd + kNodeHeaderWords) + 4 * sw;
     if (s < n) L.posof[pi[s]] = (uint8_t)s;
     const int my_moff = __shfl(moff, root), my_eoff = __shfl(eoff, root);
     wave_sync();
@@ -1041,14 +1054,14 @@ __device__ __forceinline__ bool cycle_local_children(WaveState &W, WaveLds &L, u
         const unsigned long long below = (s == 0) ? 0ULL : (am & (~0ULL >> (64 - s)));
         if (adm) {
             tmp_k[cnt + __popcll(below)] = (uint32_t)key;
-            tmp_e[cnt + __popcll(below)] = stage_entry(i, j, k, e + delta);
+            tmp_e[cnt + __popcll(below)] = stage_entry(i, j, k, e + delta, b);
         }
         cnt += __popcll(am);
     }
     best = wave_max(best);
     if (best >= 0) {
         goal_t = lex_index(n, T, best >> 12, (best >> 6) & 63, best & 63);
-        return true;
+        return kLocalDone;
     }
     wave_sync();
     // descending triple order: the rank of an entry is the number of larger
@@ -1059,37 +1072,38 @@ __device__ __forceinline__ bool cycle_local_children(WaveState &W, WaveLds &L, u
         int rank = 0;
         for (int q = 0; q < cnt; ++q) rank += (uint32_t)__builtin_amdgcn_readlane((int)mk, q) > mk ? 1 : 0;
         wave_sync();
-        if (s < cnt) L.stage[rank] = me;
+        if (s < cnt) L.stage[scount + rank] = me;
     } else {
         for (int idx = s; idx < cnt; idx += kWave) {
             const uint32_t kk = tmp_k[idx];
             int rank = 0;
             for (int q = 0; q < cnt; ++q) rank += tmp_k[q] > kk ? 1 : 0;
-            L.stage[rank] = tmp_e[idx];
+            L.stage[scount + rank] = tmp_e[idx];
         }
     }
     if (root_mode) lds_add(&L.cnt[kCntRootsLive], (unsigned long long)cnt);
     wave_sync();
-    scount = cnt;
-    return true;
+    scount += cnt;
+    return kLocalDone;
 }
 
-// Evaluates every child of L.node.  Normal mode: returns the goal triple
-// (terminal expansion) or -1 after staging/pushing admissible children.
-// Root mode: stages and pushes every live root.
+// Evaluates every child of node nd (index b of the batch in L.node) and
+// stages the admissible ones behind the scount entries already staged,
+// flushing the stage only when it would overflow.  Normal mode: returns the
+// goal triple (terminal expansion) or -1.  Root mode: stages every live root.
 template <bool kLearned>
-__device__ __forceinline__ int expand(const SearchParams &P, WaveState &W, WaveLds &L, const ConfigDesc &C,
-                      bool root_mode) {
+__device__ __forceinline__ int expand_stage(const SearchParams &P, WaveState &W, WaveLds &L, const ConfigDesc &C,
+                                            const uint32_t *nd, int b, bool root_mode, int &scount) {
     const int n = (int)C.n;
     const int sw = (int)P.spi_words;
     const int T = (int)C.triple_count;
     const int D = (int)C.max_depth;
-    const int d = (int)(L.node[1] & 0xFF);
-    const int e = (int)((L.node[1] >> 8) & 0xFF);
+    const int d = (int)(nd[1] & 0xFF);
+    const int e = (int)((nd[1] >> 8) & 0xFF);
     uint32_t sinfo;
     {
         PHASE_T0();
-        sinfo = build_posinfo(L, n, sw, W.lane);
+        sinfo = build_posinfo(L, nd, n, sw, W.lane);
         PHASE_ADD(L, kPhaseAnalyse);
     }
     // Both predicates are monotone in the odd-cycle count and a move changes it
@@ -1112,9 +1126,6 @@ __device__ __forceinline__ int expand(const SearchParams &P, WaveState &W, WaveL
         gmask = 0;
     }
     if (gmask == 0 && amask == 0) return -1;   // no child can be a goal or admitted
-    if (W.lane < kNumCounters) L.pend[W.lane] = 0;
-    wave_sync();
-    int scount = 0;
     int goal_t = -1;
     PHASE_T0();
     // When only delta = +2 can qualify, a child must hit one cycle twice (three
@@ -1146,14 +1157,33 @@ __device__ __forceinline__ int expand(const SearchParams &P, WaveState &W, WaveL
         const unsigned long long am = __ballot(adm);
         if (am) {
             const unsigned long long below = (W.lane == 0) ? 0ULL : (am & (~0ULL >> (64 - W.lane)));
-            if (adm) L.stage[scount + __popcll(below)] = stage_entry(i, j, k, e + delta);
+            if (adm) L.stage[scount + __popcll(below)] = stage_entry(i, j, k, e + delta, b);
             scount += __popcll(am);
             if (root_mode) lds_add(&L.cnt[kCntRootsLive], __popcll(am));
             if (scount > kStage - kWave) return kScanPause;
         }
         return kScanGo;
     };
-    if (!(plus2_only && cycle_local_children(W, L, sinfo, n, sw, T, e, gmask, amask, root_mode, scount, goal_t))) {
+    int local = kLocalNo;
+    if (plus2_only) {
+        local = cycle_local_children(W, L, nd, b, sinfo, n, sw, T, e, gmask, amask, root_mode, scount, goal_t);
+        if (local == kLocalFull) {
+            // the children of earlier nodes of the batch leave no room
+            wave_sync();
+            flush_stage<kLearned>(P, W, L, scount, n, sw, T, root_mode);
+            scount = 0;
+            if (W.stop) return -1;
+            local = cycle_local_children(W, L, nd, b, sinfo, n, sw, T, e, gmask, amask, root_mode, scount, goal_t);
+        }
+    }
+    if (local == kLocalNo) {
+        if (scount > kStage - kWave) {
+            // a chunk of the scan may stage up to kWave children
+            wave_sync();
+            flush_stage<kLearned>(P, W, L, scount, n, sw, T, root_mode);
+            scount = 0;
+            if (W.stop) return -1;
+        }
         int mb = -1, qb = 0;
         while (scan_children(W.colex, n, T, W.lane, mb, qb, visit) == kScanPause) {
             // stage full: flush, then resume the scan at the next chunk
@@ -1165,17 +1195,7 @@ __device__ __forceinline__ int expand(const SearchParams &P, WaveState &W, WaveL
     }
     if (W.stop) return -1;
     PHASE_ADD(L, kPhaseScan);
-    if (goal_t >= 0) return goal_t;
-    if (scount) {
-        wave_sync();
-        flush_stage<kLearned>(P, W, L, scount, n, sw, T, root_mode);
-    }
-    if (!root_mode) {
-        wave_sync();
-        if (W.lane < kNumCounters) L.cnt[W.lane] += L.pend[W.lane];
-        wave_sync();
-    }
-    return -1;
+    return goal_t;
 }
 
 __device__ __forceinline__ void switch_config(const SearchParams &P, WaveState &W, WaveLds &L, int cfg) {
@@ -1234,18 +1254,34 @@ __device__ __forceinline__ void search_body(const SearchParams &P) {
 
     while (true) {
         if (W.stop || ((++polls & 7) == 0 && should_stop(P))) break;
-        if (past_deadline(W)) {
+        if ((polls & 63) == 0 && past_deadline(W)) {
             set_watchdog(P, W);
             break;
         }
         bool have = false;
+        int batch = 1;
         if (W.hi > W.lo) {
             PHASE_T0();
-            W.hi--;
-            lds_add(&L.traffic[0], 1);
-            const uint32_t *src = stack_slot(P, W, W.hi);
-            if (W.lane < (int)P.node_words) L.node[W.lane] = src[W.lane];
+            // up to P.batch nodes from the top of the stack (loads in flight
+            // together), of which the leading ones of one configuration are
+            // taken
+            const uint32_t avail = W.hi - W.lo;
+            const uint32_t most = kLearned ? 1u : (uint32_t)P.batch;   // the learned ordering takes one
+            const int want = (int)(avail < most ? avail : most);
+            uint32_t v[kBatchMax];
+#pragma unroll
+            for (int b = 0; b < kBatchMax; ++b) {
+                v[b] = 0;
+                if (b < want && W.lane < (int)P.node_words) v[b] = stack_slot(P, W, W.hi - 1 - b)[W.lane];
+            }
+#pragma unroll
+            for (int b = 0; b < kBatchMax; ++b)
+                if (b < want && W.lane < (int)P.node_words) L.node[b][W.lane] = v[b];
             wave_sync();
+            const bool same = W.lane < want && L.node[W.lane < kBatchMax ? W.lane : 0][0] == L.node[0][0];
+            batch = __ffsll((long long)~__ballot(same)) - 1;
+            W.hi -= (uint32_t)batch;
+            lds_add(&L.traffic[0], (unsigned long long)batch);
             have = true;
             PHASE_ADD(L, kPhasePop);
         } else if (W.configs_left) {
@@ -1268,12 +1304,17 @@ __device__ __forceinline__ void search_body(const SearchParams &P) {
                     w = reinterpret_cast<const uint32_t *>(C.spi0)[W.lane - kNodeHeaderWords];
                 else if (W.lane >= kNodeHeaderWords + sw && W.lane < kNodeHeaderWords + 2 * sw)
                     w = reinterpret_cast<const uint32_t *>(C.pi0)[W.lane - kNodeHeaderWords - sw];
-                L.node[W.lane] = w;
+                L.node[0][W.lane] = w;
             }
             wave_sync();
             if (C.triple_count > 0) {
                 PHASE_T0();
-                expand<kLearned>(P, W, L, C, true);
+                int scount = 0;
+                expand_stage<kLearned>(P, W, L, C, L.node[0], 0, true, scount);
+                if (scount && !W.stop) {
+                    wave_sync();
+                    flush_stage<kLearned>(P, W, L, scount, (int)C.n, (int)P.spi_words, (int)C.triple_count, true);
+                }
                 PHASE_ADD(L, kPhaseRoot);
             }
             continue;
@@ -1354,32 +1395,59 @@ __device__ __forceinline__ void search_body(const SearchParams &P) {
         }
         if (!have) continue;
 
-        // ---- process the node in L.node
-        const int cfg = uniform((int)L.node[0]);
+        // ---- expand the nodes in L.node[0..batch) (one configuration): their
+        // children are staged together and deduplicated in one pass
+        const int cfg = uniform((int)L.node[0][0]);
         const ConfigDesc &C = P.cfgs[cfg];
-        const int d = uniform((int)(L.node[1] & 0xFF));
-        const int e = uniform((int)((L.node[1] >> 8) & 0xFF));
         {
             PHASE_T0();
             switch_config(P, W, L, cfg);
-            lds_add(&L.live, ~0ULL);   // the node just taken
+            lds_add(&L.live, (unsigned long long)(-(long long)batch));   // the nodes just taken
             const bool skip = config_found(P, cfg);
             PHASE_ADD(L, kPhaseCheck);
             if (skip) continue;
         }
-        if ((C.accept[d] >> e) & 1ULL) {
-            report_found(P, W, L, cfg, d, -1);
-            continue;
+        if (W.lane < kNumCounters) L.pend[W.lane] = 0;
+        wave_sync();
+        int scount = 0, expanded = 0;
+        bool terminal = false;
+        // the bottom node first, so that the top node's children end on top
+        for (int b = batch - 1; b >= 0 && !W.stop; --b) {
+            const uint32_t *nd = L.node[b];
+            const int d = uniform((int)(nd[1] & 0xFF));
+            const int e = uniform((int)((nd[1] >> 8) & 0xFF));
+            if ((C.accept[d] >> e) & 1ULL) {
+                report_found(P, W, nd, cfg, d, -1);
+                terminal = true;
+                break;
+            }
+            if (d >= (int)C.max_depth || !((C.reach[d] >> e) & 1ULL)) continue;
+            lds_add(&L.cnt[kCntExpansions], 1);
+            lds_add(&L.cnt[kCntGenerated], C.triple_count);
+            ++expanded;
+            const int goal = expand_stage<kLearned>(P, W, L, C, nd, b, false, scount);
+            if (goal >= 0) {
+                report_found(P, W, nd, cfg, d + 1, goal);
+                terminal = true;   // the configuration is found: staged children are dropped
+                break;
+            }
         }
-        if (d >= (int)C.max_depth || !((C.reach[d] >> e) & 1ULL)) continue;
-        lds_add(&L.cnt[kCntExpansions], 1);
-        lds_add(&L.cnt[kCntGenerated], C.triple_count);
-        const int goal = expand<kLearned>(P, W, L, C, false);
-        if (goal >= 0) report_found(P, W, L, cfg, d + 1, goal);
+        // a terminal expansion's dedup side effects are not counted
+        if (!terminal && !W.stop) {
+            if (scount) {
+                wave_sync();
+                flush_stage<kLearned>(P, W, L, scount, (int)C.n, (int)P.spi_words, (int)C.triple_count, false);
+            }
+            wave_sync();
+            if (W.lane < kNumCounters) L.cnt[W.lane] += L.pend[W.lane];
+            wave_sync();
+        }
 
         // donation: every share_mask + 1 expansions, a wave with at least two
         // nodes hands up to half of them (at most share_max) to an idle wave
-        if (P.donate && ((++iter & P.share_mask) == 0) && W.hi - W.lo >= 2) {
+        const unsigned iter0 = iter;
+        iter += (unsigned)expanded;
+        if (P.donate && ((iter0 ^ iter) & ~P.share_mask) != 0 && W.hi - W.lo >= 2) {
             PHASE_T0();
             uint32_t give = (W.hi - W.lo) / 2;
             if (give > P.share_max) give = P.share_max;
@@ -1450,16 +1518,16 @@ __global__ __launch_bounds__(64) void eval_children_kernel(const uint8_t *pi, co
             const int s = 4 * lane + b;
             if (s < n) w |= (uint32_t)spi[s] << (8 * b);
         }
-        L.node[kNodeHeaderWords + lane] = w;
+        L.node[0][kNodeHeaderWords + lane] = w;
         w = 0;
         for (int b = 0; b < 4; ++b) {
             const int s = 4 * lane + b;
             if (s < n) w |= (uint32_t)pi[s] << (8 * b);
         }
-        L.node[kNodeHeaderWords + sw + lane] = w;
+        L.node[0][kNodeHeaderWords + sw + lane] = w;
     }
     wave_sync();
-    build_posinfo(L, n, sw, lane);
+    build_posinfo(L, L.node[0], n, sw, lane);
     // same enumeration and delta as the search scan, written at t = T-1-o
     int mb = -1, qb = 0;
     scan_children(colex, n, T, lane, mb, qb, [&](bool valid, int i, int j, int k, int obase) -> int {

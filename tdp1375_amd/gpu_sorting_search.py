@@ -102,6 +102,12 @@ def set_tuning(waves_per_cu: int = 0, share_interval: int = 0, share_max: int = 
     _lib.check(_lib.load().tdp_set_tuning(waves_per_cu, share_interval, share_max, idle_backoff, max_active))
 
 
+def set_expansion_batch(nodes: int = 0) -> None:
+    """Nodes a wave expands before one dedup pass (work-sharing mode; 0 =
+    default 4)."""
+    _lib.check(_lib.load().tdp_set_expansion_batch(int(nodes)))
+
+
 def set_launch_deadline(milliseconds: int) -> None:
     """Time budget of each launch run with FLAG_SOFT_DEADLINE."""
     _lib.check(_lib.load().tdp_set_launch_deadline(int(milliseconds)))
