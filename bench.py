@@ -87,7 +87,7 @@ def cpu_baseline(wl, sample_cases: int, cap: int, threads: int):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--cases", type=int, default=4096)
     ap.add_argument("--queue-mb", type=int, default=32768)

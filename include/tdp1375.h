@@ -172,14 +172,14 @@ TDP_API int tdp_host_predicate_tables(int n, int initial_even_cycles, float min_
 /* Work-sharing launch tuning for later launches on every slot (extension;
  * no reference counterpart).  0 keeps a value's default: waves per CU = the
  * kernel's occupancy limit, donation check every 16 expansions, 4 nodes per
- * donation, idle sleep up to 16 x 127 s_sleep units, at most 512
+ * donation, idle sleep up to 16 x 127 s_sleep units, at most 1024
  * configurations per persistent launch (a larger batch runs as consecutive
  * launches).  share_interval must be a power of two. */
 TDP_API int tdp_set_tuning(int waves_per_cu, int share_interval, int share_max, int idle_backoff,
                            int max_active);
 
 /* Nodes a wave of the work-sharing mode takes from its stack and expands
- * before one dedup pass over all their children (1..4; 0 = default 4; the
+ * before one dedup pass over all their children (1..8; 0 = default 8; the
  * deterministic modes and the learned ordering always take one).  Extension;
  * no reference counterpart. */
 TDP_API int tdp_set_expansion_batch(int nodes);

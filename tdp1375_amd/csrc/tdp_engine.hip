@@ -407,9 +407,9 @@ struct Tuning {
     int share_interval = 16;  // expansions between donation checks (power of two)
     int share_max = 4;        // nodes per donation
     int idle_backoff = 16;    // longest idle sleep, x 127 s_sleep units
-    int max_active = 512;     // configurations per persistent launch
+    int max_active = 1024;    // configurations per persistent launch
     int deadline_ms = 1000;   // TDP_FLAG_SOFT_DEADLINE: time budget of each launch
-    int batch = 4;            // nodes a wave expands before one dedup pass (plain ordering)
+    int batch = 8;            // nodes a wave expands before one dedup pass (plain ordering)
 };
 std::mutex g_tuning_mutex;
 Tuning g_tuning;
@@ -1032,7 +1032,7 @@ int tdp_set_launch_deadline(int milliseconds) {
 
 int tdp_set_expansion_batch(int nodes) {
     return guarded([&] {
-        if (nodes < 0 || nodes > kExpansionBatchMax) throw std::invalid_argument("batch must be in 0..4");
+        if (nodes < 0 || nodes > kExpansionBatchMax) throw std::invalid_argument("batch must be in 0..8");
         std::lock_guard<std::mutex> lk(g_tuning_mutex);
         g_tuning.batch = nodes ? nodes : Tuning().batch;
     });

@@ -10,7 +10,7 @@
 namespace tdp {
 
 // Most nodes a wave expands before one dedup pass over all their children.
-constexpr int kExpansionBatchMax = 4;
+constexpr int kExpansionBatchMax = 8;
 
 struct SearchParams {
     const ConfigDesc *cfgs;

@@ -625,6 +625,16 @@ __device__ __forceinline__ void push_child(const SearchParams &P, WaveState &W, 
     lds_add(&L.live, 1ULL);
 }
 
+// Pushes a copy of node nd (a node of a batch that was not expanded).
+__device__ __forceinline__ void push_node(const SearchParams &P, WaveState &W, WaveLds &L, const uint32_t *nd) {
+    if (!make_room(P, W)) return;
+    uint32_t *dst = stack_slot(P, W, W.hi);
+    if (W.lane < (int)P.node_words) dst[W.lane] = nd[W.lane];
+    W.hi++;
+    lds_add(&L.traffic[1], 1);
+    lds_add(&L.live, 1ULL);
+}
+
 // Lane-parallel dedup of staged children (sorting_search.cu:261-298
 // semantics: a child already stored at a depth <= its own is dominated, one
 // stored deeper is admitted and lowers the stored depth, an unknown one is
@@ -878,6 +888,7 @@ __device__ __forceinline__ void flush_stage(const SearchParams &P, WaveState &W,
         }
         {
             PHASE_T0();
+            int bb = kLearned ? 0 : wave_max(admit ? (int)(st >> 24) : -1);
             while (pm) {
                 int src;
                 if (kLearned) {
@@ -897,8 +908,15 @@ __device__ __forceinline__ void flush_stage(const SearchParams &P, WaveState &W,
                     src = uniform(best);
                     pm &= ~(1ULL << src);
                 } else {
-                    src = __ffsll((long long)pm) - 1;
-                    pm &= pm - 1;
+                    // children of lower nodes of the batch first (highest node
+                    // index), so that the top node's children end on top
+                    unsigned long long grp = pm & __ballot((int)(st >> 24) == bb);
+                    while (grp == 0) {
+                        --bb;
+                        grp = pm & __ballot((int)(st >> 24) == bb);
+                    }
+                    src = __ffsll((long long)grp) - 1;
+                    pm &= ~(1ULL << src);
                 }
                 // the child's move from its lane's registers (no LDS reads)
                 const uint32_t cst = (uint32_t)__builtin_amdgcn_readlane((int)st, src);
@@ -1266,7 +1284,9 @@ __device__ __forceinline__ void search_body(const SearchParams &P) {
             // together), of which the leading ones of one configuration are
             // taken
             const uint32_t avail = W.hi - W.lo;
-            const uint32_t most = kLearned ? 1u : (uint32_t)P.batch;   // the learned ordering takes one
+            // The learned ordering takes one node; so does a stack more than
+            // half full.
+            const uint32_t most = (kLearned || avail * 2 > P.stack_cap) ? 1u : (uint32_t)P.batch;
             const int want = (int)(avail < most ? avail : most);
             uint32_t v[kBatchMax];
 #pragma unroll
@@ -1409,10 +1429,15 @@ __device__ __forceinline__ void search_body(const SearchParams &P) {
         }
         if (W.lane < kNumCounters) L.pend[W.lane] = 0;
         wave_sync();
-        int scount = 0, expanded = 0;
+        int scount = 0, expanded = 0, rest = 0;
         bool terminal = false;
-        // the bottom node first, so that the top node's children end on top
+        // Bottom node first.  Expanding several nodes before their children
+        // are pushed reaches states at smaller depths first (fewer improved
+        // re-expansions) and finds goals among siblings earlier; it also
+        // leaves more nodes on the stack than DFS, so once half a stage of
+        // children is staged the nodes above go back onto the stack, on top.
         for (int b = batch - 1; b >= 0 && !W.stop; --b) {
+            rest = b;
             const uint32_t *nd = L.node[b];
             const int d = uniform((int)(nd[1] & 0xFF));
             const int e = uniform((int)((nd[1] >> 8) & 0xFF));
@@ -1431,6 +1456,7 @@ __device__ __forceinline__ void search_body(const SearchParams &P) {
                 terminal = true;   // the configuration is found: staged children are dropped
                 break;
             }
+            if (scount >= kStage / 2) break;
         }
         // a terminal expansion's dedup side effects are not counted
         if (!terminal && !W.stop) {
@@ -1438,6 +1464,8 @@ __device__ __forceinline__ void search_body(const SearchParams &P) {
                 wave_sync();
                 flush_stage<kLearned>(P, W, L, scount, (int)C.n, (int)P.spi_words, (int)C.triple_count, false);
             }
+            // nodes of the batch not expanded return on top, the top one last
+            for (int b = rest - 1; b >= 0 && !W.stop; --b) push_node(P, W, L, L.node[b]);
             wave_sync();
             if (W.lane < kNumCounters) L.cnt[W.lane] += L.pend[W.lane];
             wave_sync();
