@@ -85,3 +85,39 @@ def test_batch_runs_in_rounds_of_max_active(tuned):
             if got.found[c]:
                 assert_witness(case.pi, case.spi, case.even_cycles, wl.min_rate, wl.max_depth, got.moves[c])
     T.set_tuning()
+
+
+@pytest.mark.parametrize("nodes", [1, 3, 8])
+def test_expansion_batches_keep_verdicts(tuned, nodes):
+    """Expansion batches of 1, 3 and 8 nodes (one dedup pass over the children
+    of several expanded nodes): verdicts equal the oracle's on the n=12 batch,
+    also with small stacks that hand nodes over and spill, and on 256
+    configurations of the depth-11 generator they equal those of one node at
+    a time; every witness replays."""
+    T.set_tuning()
+    T.set_expansion_batch(nodes)
+    try:
+        wl = W.workload_random_n12()
+        cases = wl.cases[:160]
+        want = _verdicts(cases, wl)
+        for budget in (3 << 20, 1 << 30):
+            T.init_slots(1, 1, budget, 1 << 20)
+            got = T.search_batch(0, [c.pi for c in cases], [c.spi for c in cases], [c.even_cycles for c in cases],
+                                 wl.min_rate, wl.max_depth, 0, flags=0)
+            assert [bool(f) for f in got.found] == want, budget
+            for c, case in enumerate(cases):
+                if got.found[c]:
+                    assert_witness(case.pi, case.spi, case.even_cycles, wl.min_rate, wl.max_depth, got.moves[c])
+        d11 = W.workload_odd_cycle_mix_d11(256)
+        T.init_slots(1, 1, 4 << 30, 1 << 24)
+        args = ([c.pi for c in d11.cases], [c.spi for c in d11.cases], [c.even_cycles for c in d11.cases],
+                d11.min_rate, d11.max_depth, 0)
+        par = T.search_batch(0, *args, flags=0)
+        T.set_expansion_batch(1)
+        one = T.search_batch(0, *args, flags=0)
+        assert list(par.found) == list(one.found)
+        for c, case in enumerate(d11.cases):
+            if par.found[c]:
+                assert_witness(case.pi, case.spi, case.even_cycles, d11.min_rate, d11.max_depth, par.moves[c])
+    finally:
+        T.set_expansion_batch(0)
