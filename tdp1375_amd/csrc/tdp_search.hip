@@ -174,9 +174,9 @@ struct ChildMove {
 
 __device__ __forceinline__ uint32_t child_spi_word(const uint32_t *pw, int q, const ChildMove &m) {
     uint32_t w = pw[q];
-    if ((m.a >> 2) == q) w = put_byte(w, m.a, m.na);
-    if ((m.b >> 2) == q) w = put_byte(w, m.b, m.nb);
-    if ((m.c >> 2) == q) w = put_byte(w, m.c, m.nc);
+    w = ((m.a >> 2) == q) ? put_byte(w, m.a, m.na) : w;
+    w = ((m.b >> 2) == q) ? put_byte(w, m.b, m.nb) : w;
+    w = ((m.c >> 2) == q) ? put_byte(w, m.c, m.nc) : w;
     return w;
 }
 
@@ -592,34 +592,34 @@ __device__ __forceinline__ bool make_room(const SearchParams &P, WaveState &W) {
 __device__ __forceinline__ void push_child(const SearchParams &P, WaveState &W, WaveLds &L, const uint32_t *pw,
                                            int n, int sw, int t, int i, int j, int k, int even, const ChildMove &m) {
     if (!make_room(P, W)) return;
-    const uint8_t *spi = reinterpret_cast<const uint8_t *>(pw + kNodeHeaderWords);
-    const uint8_t *pi = spi + 4 * sw;
+    const uint8_t *pi = reinterpret_cast<const uint8_t *>(pw + kNodeHeaderWords) + 4 * sw;
     const int d = (int)(pw[1] & 0xFF);
     uint32_t *dst = stack_slot(P, W, W.hi);
     const int lane = W.lane;
-    if (lane < (int)P.node_words) {
-        uint32_t w = 0;
-        if (lane == 0) {
-            w = pw[0];
-        } else if (lane == 1) {
-            w = (uint32_t)(d + 1) | ((uint32_t)even << 8);
-        } else if (lane < kNodeHeaderWords) {
-            w = pw[lane];
-            const int m0 = 2 * (lane - 2);
-            if (m0 == d) w = (w & 0xFFFF0000u) | (uint32_t)t;
-            else if (m0 + 1 == d) w = (w & 0xFFFFu) | ((uint32_t)t << 16);
-        } else if (lane < kNodeHeaderWords + sw) {
-            w = child_spi_word(pw + kNodeHeaderWords, lane - kNodeHeaderWords, m);
-        } else if (lane < kNodeHeaderWords + 2 * sw) {
-            const int q = lane - kNodeHeaderWords - sw;
+    // Straight-line: every lane reads its parent word and the four pi bytes
+    // its word would take (one LDS round trip), then selects its word.
+    const uint32_t own = pw[lane < kNodeHeaderWords + 2 * sw ? lane : 0];
+    const int q = lane - kNodeHeaderWords - sw;   // pi word of this lane (when 0 <= q < sw)
+    uint32_t piw = 0;
 #pragma unroll
-            for (int b = 0; b < 4; ++b) {
-                const int p = 4 * q + b;
-                if (p < n) w |= (uint32_t)pi[transposed_src_index(p, i, j, k)] << (8 * b);
-            }
-        }
-        dst[lane] = w;
+    for (int b = 0; b < 4; ++b) {
+        const int p = (4 * q + b) & 63;
+        const uint32_t v = pi[transposed_src_index(p, i, j, k)];
+        piw |= (4 * q + b < n ? v : 0u) << (8 * b);
     }
+    const int sq = lane - kNodeHeaderWords;       // spi word of this lane (when 0 <= sq < sw)
+    uint32_t spw = own;
+    spw = ((m.a >> 2) == sq) ? put_byte(spw, m.a, m.na) : spw;
+    spw = ((m.b >> 2) == sq) ? put_byte(spw, m.b, m.nb) : spw;
+    spw = ((m.c >> 2) == sq) ? put_byte(spw, m.c, m.nc) : spw;
+    const int m0 = 2 * (lane - 2);                // header word: moves m0, m0 + 1
+    uint32_t hw = own;
+    hw = (m0 == d) ? ((hw & 0xFFFF0000u) | (uint32_t)t) : hw;
+    hw = (m0 + 1 == d) ? ((hw & 0xFFFFu) | ((uint32_t)t << 16)) : hw;
+    const uint32_t w = lane == 1 ? ((uint32_t)(d + 1) | ((uint32_t)even << 8))
+                     : lane < kNodeHeaderWords ? (lane == 0 ? own : hw)
+                     : sq < sw ? spw : piw;
+    if (lane < (int)P.node_words) dst[lane] = w;
     W.hi++;
     lds_add(&L.traffic[1], 1);
     lds_add(&L.live, 1ULL);
