@@ -316,19 +316,24 @@ __device__ __forceinline__ uint32_t build_posinfo(WaveLds &L, const uint32_t *nd
     const uint8_t *pi = spi + 4 * sw;
     const int s = lane;
     const int nxt = (s < n) ? (int)spi[s] : s;
+    // m: minimum over the 2^r symbols from s on, p: the symbol 2^r steps on.
+    // Once every window has the same minimum as the next one, each holds its
+    // cycle's minimum (the windows cover the cycle): short cycles finish in
+    // fewer than the 6 rounds that lengths up to 64 need.
     int m = s, p = nxt;
-#pragma unroll
     for (int r = 0; r < 6; ++r) {
         const int mo = __shfl(m, p);
+        if (__ballot(mo != m) == 0) break;
         const int pp = __shfl(p, p);
         m = min(m, mo);
         p = pp;
     }
     const int root = m;
+    // f: steps from s to q, q: 2^r steps on or the root (where it stays)
     int f = (s == root) ? 0 : 1;
     int q = (s == root) ? s : nxt;
-#pragma unroll
     for (int r = 0; r < 6; ++r) {
+        if (__ballot(q != root) == 0) break;
         const int fq = __shfl(f, q);
         const int qq = __shfl(q, q);
         f += fq;
