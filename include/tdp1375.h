@@ -180,7 +180,7 @@ TDP_API int tdp_set_tuning(int waves_per_cu, int share_interval, int share_max, 
 
 /* Nodes a wave of the work-sharing mode takes from its stack and expands
  * before one dedup pass over all their children (1..8; 0 = default 8; the
- * deterministic modes and the learned ordering always take one).  Extension;
+ * deterministic modes always take one).  Extension;
  * no reference counterpart. */
 TDP_API int tdp_set_expansion_batch(int nodes);
 

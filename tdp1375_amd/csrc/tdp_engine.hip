@@ -703,9 +703,9 @@ void run_launch(Slot &s, const std::vector<CaseInput> &cases, uint32_t flags, bo
     P.cancel_flag = as<int>(s.cancel_flag);
     P.donate = donate ? 1 : 0;
     P.share_mask = (uint32_t)tune.share_interval - 1;
-    // the fixed expansion order of the deterministic modes and the learned
-    // ordering (which scores one node's children) take one node at a time
-    P.batch = (donate && !(flags & TDP_FLAG_LEARNED)) ? tune.batch : 1;
+    // the fixed expansion order of the deterministic modes takes one node at
+    // a time
+    P.batch = donate ? tune.batch : 1;
     P.share_max = share_max;
     P.idle_backoff = (uint32_t)tune.idle_backoff;
     P.timeout_ticks = soft ? (unsigned long long)tune.deadline_ms * 100000ULL : kernel_timeout_ticks();

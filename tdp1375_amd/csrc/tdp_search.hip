@@ -50,7 +50,7 @@ enum DedupOutcome { kDedupPending = -1, kDedupDominated = 0, kDedupImproved = 1,
 // registers (lane 0 updates them with wave-uniform values).
 struct WaveLds {
     uint32_t node[kBatchMax][48];            // the nodes being expanded (node[0] alone outside batches)
-    uint32_t posinfo[64];
+    uint32_t posinfo[kBatchMax][64];         // cycle info per position of each node of the batch
     uint32_t memb[64];                       // cycle members in spi order, cycles of length >= 2:
                                              // cycle info | position in pi << 24
     uint32_t ememb[64];                      // the members of the even cycles again, contiguous
@@ -311,7 +311,7 @@ __device__ __forceinline__ int transposed_src_index(int p, int i, int j, int k) 
 
 // Lane s handles symbol s: cycle id (min symbol) and distance to it by
 // pointer jumping over ds_bpermute, then posinfo[p] = info(pi[p]).
-__device__ __forceinline__ uint32_t build_posinfo(WaveLds &L, const uint32_t *nd, int n, int sw, int lane) {
+__device__ __forceinline__ uint32_t build_posinfo(WaveLds &L, const uint32_t *nd, int b, int n, int sw, int lane) {
     const uint8_t *spi = reinterpret_cast<const uint8_t *>(nd + kNodeHeaderWords);
     const uint8_t *pi = spi + 4 * sw;
     const int s = lane;
@@ -346,7 +346,7 @@ __device__ __forceinline__ uint32_t build_posinfo(WaveLds &L, const uint32_t *nd
     const int pos = (f == 0) ? 0 : len - f;
     const uint32_t info = cycle_info(root, pos, len);
     const int x = (s < n) ? (int)pi[s] : s;
-    L.posinfo[lane] = (uint32_t)__shfl((int)info, x);
+    L.posinfo[b][lane] = (uint32_t)__shfl((int)info, x);
     wave_sync();
     return info;   // of symbol `lane`
 }
@@ -865,7 +865,8 @@ __device__ __forceinline__ void flush_stage(const SearchParams &P, WaveState &W,
             const ConfigDesc &C = P.cfgs[cfg];
             const int e = (int)((nd[1] >> 8) & 0xFF);
             uint32_t f[kFeatWords];
-            move_features(L.posinfo[i], L.posinfo[j], L.posinfo[k], i, j, k, n, d, (int)C.max_depth,
+            const uint32_t *pinfo = L.posinfo[st >> 24];   // the child's parent
+            move_features(pinfo[i], pinfo[j], pinfo[k], i, j, k, n, d, (int)C.max_depth,
                           (int)C.e_start, ce, ce - e, f);
             const unsigned long long below = (W.lane == 0) ? 0ULL : (pm & (~0ULL >> (64 - W.lane)));
             const int ci = __popcll(below);
@@ -981,7 +982,7 @@ __device__ __forceinline__ void report_found(const SearchParams &P, WaveState &W
 // would and returns kLocalDone.
 enum { kLocalNo = 0, kLocalDone = 1, kLocalFull = 2 };
 
-__device__ __forceinline__ int cycle_local_children(WaveState &W, WaveLds &L, const uint32_t *nd, int b,
+__device__ __forceinline__ int cycle_local_children(WaveState &W, WaveLds &L, const uint32_t *nd, int b, int pb,
                                                     uint32_t sinfo, int n, int sw, int T, int e, uint32_t gmask,
                                                     uint32_t amask, bool root_mode, int &scount, int &goal_t) {
     const int s = W.lane;
@@ -1029,9 +1030,9 @@ __device__ __forceinline__ int cycle_local_children(WaveState &W, WaveLds &L, co
         uint32_t ma, mb, mc;
         if (!valid) {
             // three distinct positions, result masked below
-            ma = L.posinfo[0];
-            mb = L.posinfo[1] | (1u << 24);
-            mc = L.posinfo[2] | (2u << 24);
+            ma = L.posinfo[pb][0];
+            mb = L.posinfo[pb][1] | (1u << 24);
+            mc = L.posinfo[pb][2] | (2u << 24);
         } else if (g < KA) {
             // three members of one cycle: the cycle holding candidate g (the
             // last one starting at or before it), then its colex triple
@@ -1123,10 +1124,13 @@ __device__ __forceinline__ int expand_stage(const SearchParams &P, WaveState &W,
     const int D = (int)C.max_depth;
     const int d = (int)(nd[1] & 0xFF);
     const int e = (int)((nd[1] >> 8) & 0xFF);
+    // the learned ordering scores children at the flush from their parent's
+    // cycle info, so each node of its batches keeps its own
+    const int pb = kLearned ? b : 0;
     uint32_t sinfo;
     {
         PHASE_T0();
-        sinfo = build_posinfo(L, nd, n, sw, W.lane);
+        sinfo = build_posinfo(L, nd, pb, n, sw, W.lane);
         PHASE_ADD(L, kPhaseAnalyse);
     }
     // Both predicates are monotone in the odd-cycle count and a move changes it
@@ -1158,7 +1162,7 @@ __device__ __forceinline__ int expand_stage(const SearchParams &P, WaveState &W,
     // Visits the chunk whose lane l holds the child at descending offset
     // obase + l (lexicographic index T-1-obase-l); true ends the scan.
     auto visit = [&](bool valid, int i, int j, int k, int obase) -> int {
-        const uint32_t ia = L.posinfo[i], ib = L.posinfo[j], ic = L.posinfo[k];
+        const uint32_t ia = L.posinfo[pb][i], ib = L.posinfo[pb][j], ic = L.posinfo[pb][k];
         int delta;
         if (plus2_only) {
             const bool shared = hits_a_cycle_twice(ia, ib, ic);
@@ -1189,14 +1193,14 @@ __device__ __forceinline__ int expand_stage(const SearchParams &P, WaveState &W,
     };
     int local = kLocalNo;
     if (plus2_only) {
-        local = cycle_local_children(W, L, nd, b, sinfo, n, sw, T, e, gmask, amask, root_mode, scount, goal_t);
+        local = cycle_local_children(W, L, nd, b, pb, sinfo, n, sw, T, e, gmask, amask, root_mode, scount, goal_t);
         if (local == kLocalFull) {
             // the children of earlier nodes of the batch leave no room
             wave_sync();
             flush_stage<kLearned>(P, W, L, scount, n, sw, T, root_mode);
             scount = 0;
             if (W.stop) return -1;
-            local = cycle_local_children(W, L, nd, b, sinfo, n, sw, T, e, gmask, amask, root_mode, scount, goal_t);
+            local = cycle_local_children(W, L, nd, b, pb, sinfo, n, sw, T, e, gmask, amask, root_mode, scount, goal_t);
         }
     }
     if (local == kLocalNo) {
@@ -1289,9 +1293,8 @@ __device__ __forceinline__ void search_body(const SearchParams &P) {
             // together), of which the leading ones of one configuration are
             // taken
             const uint32_t avail = W.hi - W.lo;
-            // The learned ordering takes one node; so does a stack more than
-            // half full.
-            const uint32_t most = (kLearned || avail * 2 > P.stack_cap) ? 1u : (uint32_t)P.batch;
+            // A stack more than half full takes one node.
+            const uint32_t most = (avail * 2 > P.stack_cap) ? 1u : (uint32_t)P.batch;
             const int want = (int)(avail < most ? avail : most);
             uint32_t v[kBatchMax];
 #pragma unroll
@@ -1436,11 +1439,13 @@ __device__ __forceinline__ void search_body(const SearchParams &P) {
         wave_sync();
         int scount = 0, expanded = 0, rest = 0;
         bool terminal = false;
+        const uint32_t hi0 = W.hi;   // children flushed during the batch land above
         // Bottom node first.  Expanding several nodes before their children
         // are pushed reaches states at smaller depths first (fewer improved
         // re-expansions) and finds goals among siblings earlier; it also
-        // leaves more nodes on the stack than DFS, so once half a stage of
-        // children is staged the nodes above go back onto the stack, on top.
+        // leaves more nodes on the stack than DFS, so once the batch has
+        // produced half a stage of children the nodes above go back onto the
+        // stack, on top.
         for (int b = batch - 1; b >= 0 && !W.stop; --b) {
             rest = b;
             const uint32_t *nd = L.node[b];
@@ -1461,7 +1466,9 @@ __device__ __forceinline__ void search_body(const SearchParams &P) {
                 terminal = true;   // the configuration is found: staged children are dropped
                 break;
             }
-            if (scount >= kStage / 2) break;
+            // half a stage of children (staged, or pushed by a flush inside
+            // a wide expansion) ends the batch
+            if (scount + (int)(W.hi - hi0) >= kStage / 2) break;
         }
         // a terminal expansion's dedup side effects are not counted
         if (!terminal && !W.stop) {
@@ -1560,17 +1567,18 @@ __global__ __launch_bounds__(64) void eval_children_kernel(const uint8_t *pi, co
         L.node[0][kNodeHeaderWords + sw + lane] = w;
     }
     wave_sync();
-    build_posinfo(L, L.node[0], n, sw, lane);
+    build_posinfo(L, L.node[0], 0, n, sw, lane);
     // same enumeration and delta as the search scan, written at t = T-1-o
     int mb = -1, qb = 0;
     scan_children(colex, n, T, lane, mb, qb, [&](bool valid, int i, int j, int k, int obase) -> int {
-        const int delta = child_delta(L.posinfo[i], L.posinfo[j], L.posinfo[k]);
+        const int delta = child_delta(L.posinfo[0][i], L.posinfo[0][j], L.posinfo[0][k]);
         const int t = T - 1 - (obase + lane);
         if (valid && out) out[t] = e + delta;
         if (valid && features) {
             // the learned ordering's features of this move, as the search computes them
             uint32_t f[kFeatWords];
-            move_features(L.posinfo[i], L.posinfo[j], L.posinfo[k], i, j, k, n, d, D, e0, e + delta, delta, f);
+            move_features(L.posinfo[0][i], L.posinfo[0][j], L.posinfo[0][k], i, j, k, n, d, D, e0, e + delta, delta,
+                          f);
             for (int w = 0; w < kFeatWords; ++w) features[(size_t)t * kFeatWords + w] = f[w];
         }
         return kScanGo;
