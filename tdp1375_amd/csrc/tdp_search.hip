@@ -593,16 +593,13 @@ __device__ __forceinline__ bool make_room(const SearchParams &P, WaveState &W) {
     return false;
 }
 
-// Pushes the child (t, i, j, k, even) of the node in L.node.
-__device__ __forceinline__ void push_child(const SearchParams &P, WaveState &W, WaveLds &L, const uint32_t *pw,
-                                           int n, int sw, int t, int i, int j, int k, int even, const ChildMove &m) {
-    if (!make_room(P, W)) return;
+// Word `lane` of the child (t, i, j, k, even) of node pw.  Straight-line:
+// the lane reads its parent word and the four pi bytes its word would take
+// (one LDS round trip), then selects its word.
+__device__ __forceinline__ uint32_t child_word(const uint32_t *pw, int lane, int n, int sw, int t, int i, int j,
+                                               int k, int even, const ChildMove &m) {
     const uint8_t *pi = reinterpret_cast<const uint8_t *>(pw + kNodeHeaderWords) + 4 * sw;
     const int d = (int)(pw[1] & 0xFF);
-    uint32_t *dst = stack_slot(P, W, W.hi);
-    const int lane = W.lane;
-    // Straight-line: every lane reads its parent word and the four pi bytes
-    // its word would take (one LDS round trip), then selects its word.
     const uint32_t own = pw[lane < kNodeHeaderWords + 2 * sw ? lane : 0];
     const int q = lane - kNodeHeaderWords - sw;   // pi word of this lane (when 0 <= q < sw)
     uint32_t piw = 0;
@@ -621,14 +618,23 @@ __device__ __forceinline__ void push_child(const SearchParams &P, WaveState &W, 
     uint32_t hw = own;
     hw = (m0 == d) ? ((hw & 0xFFFF0000u) | (uint32_t)t) : hw;
     hw = (m0 + 1 == d) ? ((hw & 0xFFFFu) | ((uint32_t)t << 16)) : hw;
-    const uint32_t w = lane == 1 ? ((uint32_t)(d + 1) | ((uint32_t)even << 8))
-                     : lane < kNodeHeaderWords ? (lane == 0 ? own : hw)
-                     : sq < sw ? spw : piw;
-    if (lane < (int)P.node_words) dst[lane] = w;
+    return lane == 1 ? ((uint32_t)(d + 1) | ((uint32_t)even << 8))
+         : lane < kNodeHeaderWords ? (lane == 0 ? own : hw)
+         : sq < sw ? spw : piw;
+}
+
+// Pushes the child (t, i, j, k, even) of node pw.
+__device__ __forceinline__ void push_child(const SearchParams &P, WaveState &W, WaveLds &L, const uint32_t *pw,
+                                           int n, int sw, int t, int i, int j, int k, int even, const ChildMove &m) {
+    if (!make_room(P, W)) return;
+    const uint32_t w = child_word(pw, W.lane, n, sw, t, i, j, k, even, m);
+    uint32_t *dst = stack_slot(P, W, W.hi);
+    if (W.lane < (int)P.node_words) dst[W.lane] = w;
     W.hi++;
     lds_add(&L.traffic[1], 1);
     lds_add(&L.live, 1ULL);
 }
+
 
 // Pushes a copy of node nd (a node of a batch that was not expanded).
 __device__ __forceinline__ void push_node(const SearchParams &P, WaveState &W, WaveLds &L, const uint32_t *nd) {
@@ -895,7 +901,8 @@ __device__ __forceinline__ void flush_stage(const SearchParams &P, WaveState &W,
         {
             PHASE_T0();
             int bb = kLearned ? 0 : wave_max(admit ? (int)(st >> 24) : -1);
-            while (pm) {
+            // the next child to push, in push order
+            auto pick = [&]() -> int {
                 int src;
                 if (kLearned) {
                     // ascending score: the best-scored child is pushed last
@@ -912,7 +919,6 @@ __device__ __forceinline__ void flush_stage(const SearchParams &P, WaveState &W,
                         }
                     }
                     src = uniform(best);
-                    pm &= ~(1ULL << src);
                 } else {
                     // children of lower nodes of the batch first (highest node
                     // index), so that the top node's children end on top
@@ -922,18 +928,45 @@ __device__ __forceinline__ void flush_stage(const SearchParams &P, WaveState &W,
                         grp = pm & __ballot((int)(st >> 24) == bb);
                     }
                     src = __ffsll((long long)grp) - 1;
-                    pm &= ~(1ULL << src);
                 }
-                // the child's move from its lane's registers (no LDS reads)
-                const uint32_t cst = (uint32_t)__builtin_amdgcn_readlane((int)st, src);
-                const int ct = __builtin_amdgcn_readlane(t, src);
-                const uint32_t c1 = (uint32_t)__builtin_amdgcn_readlane((int)mv1, src);
-                const uint32_t c2 = (uint32_t)__builtin_amdgcn_readlane((int)mv2, src);
+                pm &= ~(1ULL << src);
+                return src;
+            };
+            // two children per store when a node fits in half a wave: lanes
+            // 0..31 write the first, lanes 32..63 the second
+            const bool pairs = (int)P.node_words <= kWave / 2;
+            while (pm) {
+                const int src0 = pick();
+                const bool two = pairs && pm != 0 && W.hi - W.lo + 2 <= P.stack_cap;
+                const int src1 = two ? pick() : src0;
+                const bool upper = two && W.lane >= kWave / 2;
+                // the children's moves from their lanes' registers (no LDS reads)
+                const uint32_t cst0 = (uint32_t)__builtin_amdgcn_readlane((int)st, src0);
+                const uint32_t cst1 = (uint32_t)__builtin_amdgcn_readlane((int)st, src1);
+                const int ct0 = __builtin_amdgcn_readlane(t, src0), ct1 = __builtin_amdgcn_readlane(t, src1);
+                const uint32_t a0 = (uint32_t)__builtin_amdgcn_readlane((int)mv1, src0);
+                const uint32_t a1 = (uint32_t)__builtin_amdgcn_readlane((int)mv1, src1);
+                const uint32_t b0 = (uint32_t)__builtin_amdgcn_readlane((int)mv2, src0);
+                const uint32_t b1 = (uint32_t)__builtin_amdgcn_readlane((int)mv2, src1);
+                const uint32_t cst = upper ? cst1 : cst0, c1 = upper ? a1 : a0, c2 = upper ? b1 : b0;
+                const int ct = upper ? ct1 : ct0;
                 ChildMove cm;
                 cm.a = (int)(c1 & 0xFFu); cm.b = (int)((c1 >> 8) & 0xFFu); cm.c = (int)(c1 >> 16);
                 cm.na = c2 & 0xFFu; cm.nb = (c2 >> 8) & 0xFFu; cm.nc = c2 >> 16;
-                push_child(P, W, L, L.node[cst >> 24], n, sw, ct, (int)(cst & 63u), (int)((cst >> 6) & 63u),
-                           (int)((cst >> 12) & 63u), (int)((cst >> 18) & 63u), cm);
+                if (two) {
+                    const int wl = W.lane & (kWave / 2 - 1);
+                    const uint32_t w = child_word(L.node[cst >> 24], wl, n, sw, ct, (int)(cst & 63u),
+                                                  (int)((cst >> 6) & 63u), (int)((cst >> 12) & 63u),
+                                                  (int)((cst >> 18) & 63u), cm);
+                    uint32_t *dst = stack_slot(P, W, W.hi + (upper ? 1u : 0u));
+                    if (wl < (int)P.node_words) dst[wl] = w;
+                    W.hi += 2;
+                    lds_add(&L.traffic[1], 2);
+                    lds_add(&L.live, 2ULL);
+                } else {
+                    push_child(P, W, L, L.node[cst >> 24], n, sw, ct, (int)(cst & 63u), (int)((cst >> 6) & 63u),
+                               (int)((cst >> 12) & 63u), (int)((cst >> 18) & 63u), cm);
+                }
                 if (W.stop) break;
             }
             PHASE_ADD(L, kPhasePush);
