@@ -212,6 +212,20 @@ TDP_API int tdp_mlp_train(const uint32_t *rows, const uint8_t *labels, const int
  * default 1000 ms). */
 TDP_API int tdp_set_launch_deadline(int milliseconds);
 
+/* Canonical form of `count` configurations (extension; replaces
+ * Configuration.getSignature / getEquivalentSignatures / getCanonical /
+ * hashCode, Configuration.java:53-198,243-297, for the Java caller's case
+ * keys).  Configuration c has ns[c] <= 64 symbols, spi as a one-line array
+ * spis[c*64 + x] = image of x (no fixed points) and pi as the symbols by
+ * position pis[c*64 + i].  Writes the least of its 2n equivalent signatures
+ * by CRC32 hash then content (out_signatures[c*64 + i], float), that hash
+ * (= Configuration.hashCode), the index 2*rotation + mirror of the
+ * equivalent signature it is, and the configuration it stands for
+ * (ofSignature, pi the identity) as a one-line spi. */
+TDP_API int tdp_canonical_batch(int count, const uint8_t *spis, const uint8_t *pis, const int32_t *ns,
+                                float *out_signatures, int32_t *out_hashes, int32_t *out_index,
+                                uint8_t *out_canonical_spi);
+
 /* Number of visible GPUs (0 when none). */
 TDP_API int tdp_device_count(void);
 

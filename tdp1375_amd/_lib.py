@@ -34,6 +34,7 @@ EXPORTED_SYMBOLS = (
     "tdp_search_batch", "tdp_last_launch_stats", "tdp_cancel", "tdp_filter", "tdp_eval_features",
     "tdp_filter_last_kernel_ms", "tdp_eval_children", "tdp_host_eval_children",
     "tdp_host_predicate_tables", "tdp_set_tuning", "tdp_set_launch_deadline", "tdp_set_expansion_batch",
+    "tdp_canonical_batch",
     "tdp_device_count",
     "tdp_mlp_set_weights",
     "tdp_mlp_get_weights", "tdp_mlp_init", "tdp_mlp_forward", "tdp_mlp_gradients", "tdp_mlp_train",
@@ -129,6 +130,8 @@ def load() -> ctypes.CDLL:
         L.tdp_set_tuning.argtypes = [ctypes.c_int] * 5
         L.tdp_set_launch_deadline.argtypes = [ctypes.c_int]
         L.tdp_set_expansion_batch.argtypes = [ctypes.c_int]
+        L.tdp_canonical_batch.argtypes = [ctypes.c_int, u8p, u8p, i32p, ctypes.POINTER(ctypes.c_float), i32p, i32p,
+                                          u8p]
         L.tdp_device_count.argtypes = []
         f32p = ctypes.POINTER(ctypes.c_float)
         u32p = ctypes.POINTER(ctypes.c_uint32)

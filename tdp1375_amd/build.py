@@ -20,7 +20,7 @@ OUT_DIR = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libtdp1375_jni.so")
 ARCH = os.environ.get("TDP_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["tdp_search.hip", "tdp_filter.hip", "tdp_learn.hip", "tdp_engine.hip", "jni_shim.cpp"]
+SOURCES = ["tdp_search.hip", "tdp_filter.hip", "tdp_learn.hip", "tdp_canon.hip", "tdp_engine.hip", "jni_shim.cpp"]
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-fvisibility=hidden",
           "-I", os.path.join(HERE, "..", "include")]

@@ -1,0 +1,107 @@
+"""Configuration canonicalisation on the GPU (host-side mirror).
+
+Mirrors the part of Configuration.java that keys proof cases
+(Configuration.java:53-198,243-297): a configuration is spi (a product of
+cycles, no fixed points) over pi (a cycle, the canonical 0..n-1 unless
+given); its signature labels every position of pi by the cycle of its symbol
+(labels by first appearance, oriented cycles with a rank in hundredths); the
+canonical configuration is the one of the least of the 2n signatures of
+every rotation of pi and its mirror, by CRC32 hash then content, and that
+hash is the configuration's hashCode.  The work runs batched in one launch
+(tdp_canonical_batch).
+"""
+from __future__ import annotations
+
+import ctypes
+import re
+from dataclasses import dataclass
+from typing import List, Optional, Sequence
+
+import numpy as np
+
+from . import _lib
+
+MAX_N = 64
+
+
+def parse_cycles(text: str) -> List[List[int]]:
+    """'(0 9 7)(1 5 2 6 3)' -> [[0, 9, 7], [1, 5, 2, 6, 3]] (MulticyclePermutation.java:78-88)."""
+    return [[int(x) for x in re.split(r"[,\s]+", g.strip()) if x] for g in re.findall(r"\(([^()]*?)\)", text)]
+
+
+def one_line(cycles: Sequence[Sequence[int]], n: int) -> List[int]:
+    img = list(range(n))
+    for c in cycles:
+        for t, s in enumerate(c):
+            img[s] = c[(t + 1) % len(c)]
+    return img
+
+
+def cycles_of(img: Sequence[int]) -> List[List[int]]:
+    seen, out = set(), []
+    for s in range(len(img)):
+        if s in seen:
+            continue
+        c, x = [], s
+        while x not in seen:
+            seen.add(x)
+            c.append(x)
+            x = img[x]
+        out.append(c)
+    return out
+
+
+def format_cycles(cycles: Sequence[Sequence[int]]) -> str:
+    return "".join("(" + " ".join(str(s) for s in c) + ")" for c in cycles)
+
+
+@dataclass
+class Canonical:
+    signature: np.ndarray       # float32[n]: the canonical signature
+    hash: int                   # Configuration.hashCode
+    index: int                  # 2 * rotation + mirror of the equivalent signature it is
+    spi: List[int]              # canonical configuration, one-line (pi the identity)
+
+    @property
+    def cycles(self) -> List[List[int]]:
+        return cycles_of(self.spi)
+
+
+def canonical_batch(spis: Sequence[Sequence[int]], pis: Optional[Sequence[Sequence[int]]] = None) -> List[Canonical]:
+    """Canonical forms of many configurations in one launch.  spis[c] is a
+    one-line spi (image of every symbol); pis[c] the symbols of pi by position
+    (default 0..n-1)."""
+    count = len(spis)
+    if count == 0:
+        return []
+    S = np.zeros((count, MAX_N), np.uint8)
+    P = np.zeros((count, MAX_N), np.uint8)
+    ns = np.zeros(count, np.int32)
+    for c, spi in enumerate(spis):
+        n = len(spi)
+        if n > MAX_N:
+            raise _lib.TdpError(_lib.TDP_ERR_INVALID_ARGUMENT, "configuration size must be in 1..64")
+        ns[c] = n
+        S[c, :n] = np.asarray(spi, np.uint8)
+        P[c, :n] = np.asarray(pis[c] if pis is not None else range(n), np.uint8)
+    sig = np.zeros((count, MAX_N), np.float32)
+    h = np.zeros(count, np.int32)
+    idx = np.zeros(count, np.int32)
+    out = np.zeros((count, MAX_N), np.uint8)
+    u8p = ctypes.POINTER(ctypes.c_uint8)
+    i32p = ctypes.POINTER(ctypes.c_int32)
+    _lib.check(_lib.load().tdp_canonical_batch(count, S.ctypes.data_as(u8p), P.ctypes.data_as(u8p),
+                                               ns.ctypes.data_as(i32p),
+                                               sig.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                                               h.ctypes.data_as(i32p), idx.ctypes.data_as(i32p),
+                                               out.ctypes.data_as(u8p)))
+    return [Canonical(sig[c, :ns[c]].copy(), int(h[c]), int(idx[c]), [int(x) for x in out[c, :ns[c]]])
+            for c in range(count)]
+
+
+def canonical(spi: str) -> Canonical:
+    """Canonical form of one configuration given as a cycle string over the
+    canonical pi (new Configuration(spi).getCanonical())."""
+    cycles = parse_cycles(spi)
+    n = sum(len(c) for c in cycles)
+    return canonical_batch([one_line(cycles, n)])[0]

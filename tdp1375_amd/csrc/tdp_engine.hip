@@ -27,6 +27,7 @@
 #include "../../include/tdp1375.h"
 #include "tdp_common.h"
 #include "tdp_filter.h"
+#include "tdp_canon.h"
 #include "tdp_search.h"
 #include "tdp_learn.h"
 
@@ -1213,6 +1214,50 @@ int tdp_eval_features(const uint8_t *pi, const uint8_t *spi, int n, int e0, int 
         HIP_CHECK(hipGetLastError());
         HIP_CHECK(hipDeviceSynchronize());
         HIP_CHECK(hipMemcpy(out, d_out.get(), (size_t)T * 16, hipMemcpyDeviceToHost));
+    });
+}
+
+int tdp_canonical_batch(int count, const uint8_t *spis, const uint8_t *pis, const int32_t *ns, float *out_signatures,
+                        int32_t *out_hashes, int32_t *out_index, uint8_t *out_canonical_spi) {
+    return guarded([&] {
+        if (count < 0) throw std::invalid_argument("count must be non-negative");
+        if (count == 0) return;
+        if (!spis || !pis || !ns || !out_signatures || !out_hashes || !out_index || !out_canonical_spi)
+            throw std::invalid_argument("null argument");
+        // Configuration.java takes spi without fixed points (every symbol in a
+        // cycle of the string form) and pi a cycle over the same symbols
+        for (int c = 0; c < count; ++c) {
+            const int n = ns[c];
+            if (n < 1 || n > kMaxCanonN) throw std::invalid_argument("configuration size must be in 1..64");
+            const uint8_t *spi = spis + (size_t)c * kMaxCanonN, *pi = pis + (size_t)c * kMaxCanonN;
+            uint64_t seen_s = 0, seen_p = 0;
+            for (int x = 0; x < n; ++x) {
+                if (spi[x] >= n || pi[x] >= n) throw std::invalid_argument("spi and pi must be permutations of 0..n-1");
+                if (spi[x] == x && n > 1) throw std::invalid_argument("spi has a fixed point");
+                seen_s |= 1ULL << spi[x];
+                seen_p |= 1ULL << pi[x];
+            }
+            const uint64_t all = n == 64 ? ~0ULL : ((1ULL << n) - 1);
+            if (seen_s != all || seen_p != all) throw std::invalid_argument("spi and pi must be permutations of 0..n-1");
+        }
+        const size_t rows = (size_t)count * kMaxCanonN;
+        DevPtr d_spi = make_dev(rows, "canon.spi"), d_pi = make_dev(rows, "canon.pi");
+        DevPtr d_n = make_dev((size_t)count * 4, "canon.n");
+        DevPtr d_sig = make_dev(rows * 4, "canon.signature"), d_hash = make_dev((size_t)count * 4, "canon.hash");
+        DevPtr d_idx = make_dev((size_t)count * 4, "canon.index"), d_out = make_dev(rows, "canon.spi_out");
+        HIP_CHECK(hipMemcpy(d_spi.get(), spis, rows, hipMemcpyHostToDevice));
+        HIP_CHECK(hipMemcpy(d_pi.get(), pis, rows, hipMemcpyHostToDevice));
+        HIP_CHECK(hipMemcpy(d_n.get(), ns, (size_t)count * 4, hipMemcpyHostToDevice));
+        HIP_CHECK(hipMemset(d_sig.get(), 0, rows * 4));
+        HIP_CHECK(hipMemset(d_out.get(), 0, rows));
+        launch_canon(as<uint8_t>(d_spi), as<uint8_t>(d_pi), as<int32_t>(d_n), count, as<float>(d_sig),
+                     as<int32_t>(d_hash), as<int32_t>(d_idx), as<uint8_t>(d_out), nullptr);
+        HIP_CHECK(hipGetLastError());
+        HIP_CHECK(hipDeviceSynchronize());
+        HIP_CHECK(hipMemcpy(out_signatures, d_sig.get(), rows * 4, hipMemcpyDeviceToHost));
+        HIP_CHECK(hipMemcpy(out_hashes, d_hash.get(), (size_t)count * 4, hipMemcpyDeviceToHost));
+        HIP_CHECK(hipMemcpy(out_index, d_idx.get(), (size_t)count * 4, hipMemcpyDeviceToHost));
+        HIP_CHECK(hipMemcpy(out_canonical_spi, d_out.get(), rows, hipMemcpyDeviceToHost));
     });
 }
 
