@@ -105,3 +105,40 @@ def canonical(spi: str) -> Canonical:
     cycles = parse_cycles(spi)
     n = sum(len(c) for c in cycles)
     return canonical_batch([one_line(cycles, n)])[0]
+
+
+def main(argv=None):
+    """Times tdp_canonical_batch on seeded fixed-point-free configurations:
+    python -m tdp1375_amd.canonical --count 4096 --n 9:36"""
+    import argparse
+    import json
+    import time
+
+    ap = argparse.ArgumentParser(description="batched canonicalisation timing")
+    ap.add_argument("--count", type=int, default=4096)
+    ap.add_argument("--n", default="9:36", help="lo:hi symbols per configuration")
+    ap.add_argument("--repeat", type=int, default=3)
+    ap.add_argument("--seed", type=int, default=5)
+    args = ap.parse_args(argv)
+    lo, hi = (int(x) for x in args.n.split(":"))
+    rng = np.random.default_rng(args.seed)
+    spis = []
+    while len(spis) < args.count:
+        n = int(rng.integers(lo, hi + 1))
+        p = rng.permutation(n)
+        if n > 1 and not np.any(p == np.arange(n)):
+            spis.append([int(x) for x in p])
+    canonical_batch(spis[:64])   # warm-up
+    times = []
+    for _ in range(args.repeat):
+        t0 = time.perf_counter()
+        out = canonical_batch(spis)
+        times.append(time.perf_counter() - t0)
+    best = min(times)
+    print(json.dumps({"workload": f"canonicalise {args.count} configurations, n {lo}..{hi}", "count": args.count,
+                      "seconds_per_call": best, "configurations_per_s": args.count / best,
+                      "distinct_hashes": len({r.hash for r in out})}), flush=True)
+
+
+if __name__ == "__main__":
+    main()

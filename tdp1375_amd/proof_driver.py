@@ -250,3 +250,13 @@ def sort_batch(slot: int, configs: Sequence[Cycles], max_ratio: float, max_depth
                 pi = list(range(len(spis[c])))
                 moves[c] = [tuple(int(x) for x in m) for m in positions_to_symbols(pi, res.moves[q])]
     return LadderResult(found, tried, moves, launches)
+
+
+def write_cases(path: str, configs: Sequence[Cycles], result: LadderResult) -> int:
+    """Writes the configurations sort_batch found sequences for as a case file
+    (``spi->[(a b c), ...]``, the format of the shipped cases-*.txt that
+    AbstractSbtAlgorithm.loadSortings reads); returns the number of lines."""
+    from .case_files import from_results, write_case_file
+    lines = from_results(configs, result.moves)
+    write_case_file(path, lines)
+    return len(lines)

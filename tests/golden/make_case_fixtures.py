@@ -4,10 +4,13 @@ The reference ships its proof outputs as text lines ``spi->[(a b c), ...]``
 (src/main/resources/cases/cases-comb.txt, cases-3,2.txt,
 cases-oriented-7cycle.txt).  This script reads those data files as text and
 stores, per line, the configuration's cycles and the witness symbol triples.
-No reference code is run.  Run from the repository root:
+It also records each file's SHA-256, size and whether it ends with a
+newline (case_files_meta.json), so that the case-file writer can be checked
+byte for byte.  No reference code is run.  Run from the repository root:
 
     python tests/golden/make_case_fixtures.py /root/reference
 """
+import hashlib
 import json
 import os
 import re
@@ -28,7 +31,11 @@ def parse_cycles(text):
 
 def main(ref_root):
     out = {}
+    files = {}
     for key, rel in FILES.items():
+        raw = open(os.path.join(ref_root, rel), "rb").read()
+        files[key] = {"sha256": hashlib.sha256(raw).hexdigest(), "bytes": len(raw),
+                      "trailing_newline": raw.endswith(b"\n")}
         entries = []
         with open(os.path.join(ref_root, rel)) as fh:
             for line in fh:
@@ -48,6 +55,9 @@ def main(ref_root):
     dst = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cases.json")
     with open(dst, "w") as fh:
         json.dump(out, fh, separators=(",", ":"))
+    # byte-level facts of the files, for the case-file writer's round trip
+    with open(os.path.join(os.path.dirname(dst), "case_files_meta.json"), "w") as fh:
+        json.dump(files, fh, indent=1)
     print({k: len(v) for k, v in out.items()}, "->", dst)
 
 
