@@ -171,7 +171,7 @@ TDP_API int tdp_host_predicate_tables(int n, int initial_even_cycles, float min_
 
 /* Work-sharing launch tuning for later launches on every slot (extension;
  * no reference counterpart).  0 keeps a value's default: waves per CU = the
- * kernel's occupancy limit, donation check every 16 expansions, 4 nodes per
+ * kernel's occupancy limit, donation check every 16 expansions, 8 nodes per
  * donation, idle sleep up to 16 x 127 s_sleep units, at most 1024
  * configurations per persistent launch (a larger batch runs as consecutive
  * launches).  share_interval must be a power of two. */

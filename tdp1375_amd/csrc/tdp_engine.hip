@@ -406,7 +406,7 @@ uint64_t splitmix(uint64_t &s) {
 struct Tuning {
     int waves_per_cu = 0;     // default: the occupancy limit of search_kernel
     int share_interval = 16;  // expansions between donation checks (power of two)
-    int share_max = 4;        // nodes per donation
+    int share_max = 8;        // nodes per donation
     int idle_backoff = 16;    // longest idle sleep, x 127 s_sleep units
     int max_active = 1024;    // configurations per persistent launch
     int deadline_ms = 1000;   // TDP_FLAG_SOFT_DEADLINE: time budget of each launch
@@ -646,9 +646,12 @@ void run_launch(Slot &s, const std::vector<CaseInput> &cases, uint32_t flags, bo
     if (!(flags & TDP_FLAG_SINGLE_WAVE) && tune.waves_per_cu > 0)
         waves = std::min(waves, std::max(1, tune.waves_per_cu * dev.cus));
     uint32_t share_max = donate ? (uint32_t)tune.share_max : 0;
+    // the inboxes take at most a quarter of the budget: smaller hand-offs
+    // first, and a budget too small even for one node each runs without
+    // sharing
+    while (share_max > 1 && (size_t)waves * share_max * node_bytes * 4 > s.queue_bytes) share_max /= 2;
     size_t inbox_bytes = (size_t)waves * share_max * node_bytes;
     if (donate && inbox_bytes * 4 > s.queue_bytes) {
-        // a budget this small runs without sharing
         donate = false;
         share_max = 0;
         inbox_bytes = 0;
