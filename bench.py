@@ -99,6 +99,9 @@ def main():
     ap.add_argument("--ordering", choices=("plain", "learned"), default="plain",
                     help="learned: order children by the trained MLP in --model")
     ap.add_argument("--model", default=os.path.join(ROOT, "tdp1375_amd", "data", "ordering_mlp_d11.npz"))
+    ap.add_argument("--tuning", default=os.path.join(ROOT, "tdp1375_amd", "data", "tuning.json"),
+                    help="autotuner cache (python -m tdp1375_amd.autotune); its entry for this workload, "
+                         "if any, is applied; '' runs the library defaults")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -122,6 +125,8 @@ def main():
     wl = W.workload_odd_cycle_mix_d11(args.cases)
     cases = wl.cases
     T.init_slots(1, 1, args.queue_mb << 20, 1 << args.dedup_log2)
+    from tdp1375_amd import autotune
+    tuning = (autotune.apply(args.tuning, wl.name) if args.tuning else None) or dict(autotune.DEFAULT)
     pis = [c.pi for c in cases]
     spis = [c.spi for c in cases]
     e0s = [c.even_cycles for c in cases]
@@ -182,7 +187,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": wl.name, "cases": len(cases), "n_range": [9, 36], "max_ratio": wl.max_ratio,
                    "max_depth": wl.max_depth, "dedup_table_size": 1 << args.dedup_log2,
-                   "queue_mb": args.queue_mb, "ordering": args.ordering, "found": int(verdicts.sum())},
+                   "queue_mb": args.queue_mb, "ordering": args.ordering, "found": int(verdicts.sum()),
+                   "tuning": tuning},
         "cases_per_hour": cases_per_hour,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
