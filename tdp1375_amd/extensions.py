@@ -1,0 +1,518 @@
+"""Extension generators: the producers of search batches, and a batched
+level-by-level SortOrExtend exploration over them.
+
+Generators (host code, as in the reference; exact float32 arithmetic where
+the reference computes in float, float64 where it computes in double):
+
+* ``type1_extensions`` / ``type2_extensions`` / ``type3_extensions``
+  (Extensions.java:95-203): a new unoriented 3-cycle with one symbol in an
+  open gate; a new 3-cycle anywhere in a full configuration; an unoriented
+  cycle grown by two symbols, either staying unoriented (at most two open
+  gates after) or, for a 3-cycle, turned into an oriented 5-cycle whose
+  symbols are in the cyclic order 1,3,5,2,4 of its ranks.
+* ``combination_extensions`` (Combinations.java:64-94,183-199): a bad small
+  component inserted at every position of the signature.
+
+Each returns ``(info, cycles)`` pairs in the reference's order, with its info
+string; configurations have pi the identity (``Configuration.ofSignature``).
+
+``explore`` runs the SortOrExtend tree (SortOrExtend.java:51-87 with the
+extend overrides Extensions.java:313-325 and Combinations.java:168-181) one
+level at a time instead of one ForkJoin task per configuration: every level's
+configurations are canonicalised in one GPU launch (tdp_canonical_batch), the
+canonical forms not seen before are searched with the batched depth ladder
+(one persistent launch per rung), and every configuration whose canonical
+form has no sorting is extended into the next level.
+"""
+from __future__ import annotations
+
+import math
+import time
+from dataclasses import dataclass, field
+from typing import Callable, Dict, List, Optional, Sequence, Set, Tuple
+
+import numpy as np
+
+F32 = np.float32
+F64 = np.float64
+Cycles = List[List[int]]
+
+
+def parse(text: str) -> Cycles:
+    """'(0,4,2)(1,5,3)' or '(0 4 2)(1 5 3)' -> [[0, 4, 2], [1, 5, 3]]."""
+    out = []
+    for part in text.replace(",", " ").split(")"):
+        part = part.strip().lstrip("(")
+        if part:
+            out.append([int(x) for x in part.split()])
+    return out
+
+
+# Configuration.java roots of the two searches.
+ORIENTED_5_CYCLE = parse("(0,3,1,4,2)")
+INTERLEAVING_PAIR = parse("(0,4,2)(1,5,3)")
+INTERSECTING_PAIR = parse("(0,3,1)(2,5,4)")
+NECKLACE_SIZE_4 = parse("(0,10,2)(1,5,3)(4,8,6)(7,11,9)")
+TWISTED_NECKLACE_SIZE_4 = parse("(0,7,5)(1,11,9)(2,6,4)(3,10,8)")
+NECKLACE_SIZE_5 = parse("(0,4,2)(1,14,12)(3,7,5)(6,10,8)(9,13,11)")
+NECKLACE_SIZE_6 = parse("(0,16,2)(1,5,3)(4,8,6)(7,11,9)(10,14,12)(13,17,15)")
+
+# Combinations.java:44-46, with the info strings of :77-88
+BAD_SMALL_COMPONENTS = [
+    (ORIENTED_5_CYCLE, "bad oriented 5-cycle"),
+    (INTERLEAVING_PAIR, "unoriented interleaving pair"),
+    (NECKLACE_SIZE_4, "unoriented necklace of size 4"),
+    (TWISTED_NECKLACE_SIZE_4, "unoriented twisted necklace of size 4"),
+    (NECKLACE_SIZE_5, "unoriented necklace of size 5"),
+    (NECKLACE_SIZE_6, "unoriented necklace of size 6"),
+]
+# Extensions.java:41-46
+DFS_ROOTS = [ORIENTED_5_CYCLE, INTERLEAVING_PAIR, INTERSECTING_PAIR]
+
+
+def size(cycles: Cycles) -> int:
+    return sum(len(c) for c in cycles)
+
+
+def signature(cycles: Cycles, n: Optional[int] = None) -> np.ndarray:
+    """Configuration.signature with pi the identity (Configuration.java:53-101):
+    the label of each position's cycle by first appearance, plus rank/100 in
+    float32 for an oriented cycle (ranks along the cycle from its smallest
+    symbol, the first of its symbols along pi)."""
+    n = size(cycles) if n is None else n
+    p, cid = _one_line(cycles, n)
+    sig = np.zeros(n, F32)
+    label: Dict[int, int] = {}
+    for s in range(n):
+        label.setdefault(int(cid[s]), len(label) + 1)
+    inv = (n - np.arange(n)) % n                         # index of each symbol in pi^-1
+    for k, c in enumerate(cycles):
+        lab = F32(label[k])
+        ic = inv[c]
+        if int(np.count_nonzero(ic > np.roll(ic, -1))) <= 1:   # in the cyclic order of pi^-1
+            sig[c] = lab
+        else:
+            m = c.index(min(c))
+            for r in range(len(c)):
+                sig[c[(m + r) % len(c)]] = F32(lab + F32(F32(r + 1) / F32(100)))
+    return sig
+
+
+def _one_line(cycles: Cycles, n: int):
+    p = np.arange(n)
+    cid = np.full(n, -1)
+    for k, c in enumerate(cycles):
+        p[c] = np.roll(c, -1)
+        cid[c] = k
+    return p, cid
+
+
+def of_signature(sig: Sequence[float]) -> Cycles:
+    """Configuration.ofSignature (Configuration.java:102-154): per label, the
+    positions from last to first, or, when the label has fractional entries,
+    those positions in the order of their ranks round((v - label) * 100) in
+    float32; cycles in label order."""
+    sig = np.asarray(sig, F32)
+    n = len(sig)
+    lab = sig.astype(np.int64)
+    frac = np.fmod(sig, F32(1)) > 0
+    rank = np.floor((sig - lab.astype(F32)) * F32(100) + F32(0.5)).astype(np.int64)
+    order = np.lexsort((-np.arange(n), lab))
+    cut = np.flatnonzero(np.diff(lab[order])) + 1
+    out = []
+    for grp in np.split(order, cut):
+        f = frac[grp]
+        if f.any():
+            g = grp[f]
+            out.append(g[np.argsort(rank[g], kind="stable")].tolist())
+        else:
+            out.append(grp.tolist())
+    return out
+
+
+def open_gates(cycles: Cycles, n: Optional[int] = None) -> Set[int]:
+    """CommonOperations.getOpenGates with pi the identity, for all pairs at
+    once: a of the pair (a, spi(a)) is an open gate unless a pair (c, spi(c))
+    of another cycle has a, c, spi(a), spi(c) in the cyclic order of pi^-1,
+    or a symbol c of its own cycle has a, spi(a), c in the cyclic order of pi
+    (at most one descent, taken cyclically)."""
+    n = size(cycles) if n is None else n
+    p = np.arange(n)
+    cid = np.full(n, -1)
+    for k, c in enumerate(cycles):
+        if len(c) > 1:
+            p[c] = np.roll(c, -1)
+            cid[c] = k
+    A = np.flatnonzero(cid >= 0)
+    if A.size == 0:
+        return set()
+    B = p[A]
+    inv = (n - np.arange(n)) % n
+    ia, ib = inv[A][:, None], inv[B][:, None]
+    ic, idd = inv[A][None, :], inv[B][None, :]
+    desc4 = (ia > ic).astype(np.int8) + (ic > ib) + (ib > idd) + (idd > ia)
+    same = cid[A][:, None] == cid[A][None, :]
+    crossing = ((desc4 <= 1) & ~same).any(axis=1)
+    a_, b_, c_ = A[:, None], B[:, None], A[None, :]
+    desc3 = (a_ > b_).astype(np.int8) + (b_ > c_) + (c_ > a_)
+    triple = ((desc3 <= 1) & same & (c_ != a_) & (c_ != b_)).any(axis=1)
+    return set(A[~crossing & ~triple].tolist())
+
+
+def is_full(cycles: Cycles, n: Optional[int] = None) -> bool:
+    return not open_gates(cycles, n)
+
+
+def _split(rows: List[List[int]], lens: List[int]) -> List[Cycles]:
+    offs = np.concatenate([[0], np.cumsum(lens)]).tolist()
+    spans = list(zip(offs[:-1], offs[1:]))
+    return [[r[i:j] for i, j in spans] for r in rows]
+
+
+def _new_3cycle_extensions(cycles: Cycles, infos: List[str], triples: List[Tuple[int, int, int]]
+                           ) -> List[Tuple[str, Cycles]]:
+    """Insertion of a new label (the largest) before sorted original positions
+    p0 <= p1 <= p2 of the signature (Extensions.unorientedExtension,
+    :281-290), for all triples at once: ofSignature gives the parent's cycles
+    in label order with every symbol moved past the insertions before it, then
+    the new cycle, its positions from last to first."""
+    if not triples:
+        return []
+    sig = signature(cycles)
+    n = len(sig)
+    base = of_signature(sig)
+    flat = np.concatenate([np.asarray(c) for c in base])
+    T = np.asarray(triples)
+    ar = np.arange(n)[None, :]
+    M = ar + (ar >= T[:, 0:1]) + (ar >= T[:, 1:2]) + (ar >= T[:, 2:3])
+    rows = _split(M[:, flat].tolist(), [len(c) for c in base])
+    return [(info, ext + [[p2 + 2, p1 + 1, p0]]) for info, ext, (p0, p1, p2) in zip(infos, rows, triples)]
+
+
+def type1_extensions(cycles: Cycles) -> List[Tuple[str, Cycles]]:
+    """Extensions.java:95-116: for every open gate i and b <= c, a new 3-cycle
+    before positions i, b, c (not all equal)."""
+    n = size(cycles)
+    gates = open_gates(cycles, n)
+    infos, triples = [], []
+    for i in range(n):
+        if i in gates:
+            for b in range(n):
+                for c in range(b, n):
+                    if not (i == b == c):
+                        infos.append(f"a={i} b={b} c={c}")
+                        triples.append(tuple(sorted((i, b, c))))
+    return _new_3cycle_extensions(cycles, infos, triples)
+
+
+def type2_extensions(cycles: Cycles) -> List[Tuple[str, Cycles]]:
+    """Extensions.java:121-144: a full configuration gets a new 3-cycle before
+    every a <= b <= c (not all equal)."""
+    n = size(cycles)
+    if open_gates(cycles, n):
+        return []
+    infos, triples = [], []
+    for a in range(n):
+        for b in range(a, n):
+            for c in range(b, n):
+                if not (a == b == c):
+                    infos.append(f"a={a} b={b} c={c}")
+                    triples.append((a, b, c))
+    return _new_3cycle_extensions(cycles, infos, triples)
+
+
+def _remains_unoriented(indexes: List[int], *new) -> bool:
+    """Extensions.remainsUnoriented (:255-270): every new index falls in the
+    same gap between consecutive positions of the cycle."""
+    gaps = set()
+    m = len(indexes)
+    for x in new:
+        for t in range(m):
+            left, right = indexes[t], indexes[(t + 1) % m]
+            if (left < x <= right) or (right < left and (left < x or x <= right)):
+                gaps.add((left, right))
+    return len(gaps) == 1
+
+
+def _turn_oriented(cycle: List[int], label: int, sig: np.ndarray, a: int, b: int) -> np.ndarray:
+    """Extensions.extend (:230-253): the 3-cycle's symbols from its largest get
+    label + 0.01, 0.02, 0.03 (computed in double, stored as float), then
+    label + 0.05 is inserted before position a and label + 0.04 before b."""
+    m = cycle.index(max(cycle))
+    rot = cycle[m:] + cycle[:m]
+    out = sig.copy()
+    for t in range(len(rot) - 1, -1, -1):
+        out[rot[t]] = F32(F64(out[rot[t]]) + 0.01 * (t + 1))
+    first = F32(F32(label) + F32(0.05))
+    second = F32(F32(label) + F32(F32(0.05) - F32(0.01)))
+    return np.insert(out, [a, b], [first, second]).astype(F32)
+
+
+_RANK_ORDER = (0.01, 0.03, 0.05, 0.02, 0.04)
+
+
+def _in_rank_order(elements: np.ndarray, other: np.ndarray) -> bool:
+    """Extensions.areSymbolsInCyclicOrder(float[], float[]) (:205-228): from
+    the first element equal to other[0], once around the circle and back to
+    it, every element with other's label must equal other[0], other[1], ...
+    in turn (the grown cycle has as many elements as other, so the first such
+    element decides); true as well when no element equals other[0]."""
+    hit = np.flatnonzero(elements == other[0])
+    if hit.size == 0:
+        return True
+    ring = np.roll(elements, -int(hit[0])).tolist()
+    oth = other.tolist()
+    m = len(oth)
+    nxt = 0
+    for e in ring + ring[:1]:
+        o = oth[nxt % m]
+        if math.floor(e) == math.floor(o):
+            if e != o:
+                return False
+            nxt += 1
+            if nxt > m:
+                return True
+    return True
+
+
+def _open_gate_counts(perms: np.ndarray, cid: np.ndarray) -> np.ndarray:
+    """Number of open gates (as in open_gates) of each of T configurations of
+    N symbols at once: perms (T, N) one-line spi, cid (T, N) cycle index of
+    each symbol (-1 for a fixed point)."""
+    T, N = perms.shape
+    inv = (N - np.arange(N)) % N
+    ia = inv[None, :, None]
+    ib = inv[perms][:, :, None]
+    ic = inv[None, None, :]
+    idd = inv[perms][:, None, :]
+    desc4 = (ia > ic).astype(np.int8) + (ic > ib) + (ib > idd) + (idd > ia)
+    same = cid[:, :, None] == cid[:, None, :]
+    live = cid >= 0
+    pair = live[:, :, None] & live[:, None, :]
+    crossing = ((desc4 <= 1) & ~same & pair).any(axis=2)
+    a_ = np.arange(N)[None, :, None]
+    b_ = perms[:, :, None]
+    c_ = np.arange(N)[None, None, :]
+    desc3 = (a_ > b_).astype(np.int8) + (b_ > c_) + (c_ > a_)
+    triple = ((desc3 <= 1) & same & (c_ != a_) & (c_ != b_) & pair).any(axis=2)
+    return (live & ~crossing & ~triple).sum(axis=1)
+
+
+def type3_extensions(cycles: Cycles) -> List[Tuple[str, Cycles]]:
+    """Extensions.java:149-203: an unoriented cycle grown by two symbols
+    before positions a <= b; kept when it stays unoriented and leaves at most
+    two open gates, or, for a 3-cycle that turns oriented, when its ranks lie
+    in the cyclic order 1, 3, 5, 2, 4 along pi (otherwise the configuration
+    accepts a 2-move).
+
+    The turned cycle's values are label + 0.01, 0.02, 0.03 (float of a double
+    sum) on its symbols from the largest, and label + 0.05, label + 0.04
+    (float sums) inserted before a and b; for every label up to 199 these
+    equal the float sums label + 0.01f ... 0.05f they are compared with, so
+    the comparison of Extensions.java:205-228 is the cyclic order of those
+    five positions, and ofSignature orders the cycle by rank: the three
+    symbols from the largest, then b + 1, then a."""
+    sig = signature(cycles)
+    n = len(sig)
+    lab = sig.astype(np.int64)
+    frac = np.fmod(sig, F32(1)) > 0
+    base = of_signature(sig)
+    cycle_of = {}
+    for c in cycles:
+        for s in c:
+            cycle_of[s] = c
+    entries = []          # (info, ext, needs the open-gate test)
+    for label in range(1, len(cycles) + 1):
+        where = np.flatnonzero(lab == label)
+        if frac[where].any():
+            continue
+        pos = where.tolist()
+        cyc = cycle_of[pos[0]]
+        m0 = cyc.index(min(cyc))
+        cyc_text = "(" + " ".join(str(s) for s in cyc[m0:] + cyc[:m0]) + ")"
+        k = label - 1                                   # its index in base (label order)
+        mx = cyc.index(max(cyc))
+        rot = cyc[mx:] + cyc[:mx]
+        for a in range(n):
+            for b in range(a, n):
+                stays = _remains_unoriented(pos, a, b)
+                if not stays and len(pos) != 3:
+                    continue
+                shift = lambda x: x + (x >= a) + (x >= b)
+                ext = [[shift(x) for x in c] for c in base]
+                if stays:
+                    ext[k] = sorted(ext[k] + [a, b + 1], reverse=True)
+                    entries.append((f"a={a} b={b}, extended cycle: {cyc_text}", ext, True))
+                else:
+                    q0, q1, q2 = (shift(x) for x in rot)
+                    order = [q0, q2, a, q1, b + 1]
+                    d = [(x - q0) % (n + 2) for x in order]
+                    if d[1] < d[2] < d[3] < d[4]:
+                        ext[k] = [q0, q1, q2, b + 1, a]
+                        entries.append((f"a={a} b={b}, extended cycle: {cyc_text}, turn oriented", ext, False))
+    test = [e for e in entries if e[2]]
+    keep = {}
+    if test:
+        N = n + 2
+        rows_p, rows_c = [], []
+        for _, ext, _ in test:
+            perm = list(range(N))
+            cid = [-1] * N
+            for ci, c in enumerate(ext):
+                for t in range(len(c)):
+                    perm[c[t]] = c[t + 1 - len(c)]
+                    cid[c[t]] = ci
+            rows_p.append(perm)
+            rows_c.append(cid)
+        counts = _open_gate_counts(np.asarray(rows_p), np.asarray(rows_c))
+        keep = {id(e[1]): int(g) <= 2 for e, g in zip(test, counts)}
+    return [(info, ext) for info, ext, gated in entries if not gated or keep[id(ext)]]
+
+
+def dfs_extensions(cycles: Cycles) -> List[Tuple[str, Cycles]]:
+    """SortOrExtendExtensions.extend (Extensions.java:319-324): type 1, 2, 3."""
+    return type1_extensions(cycles) + type2_extensions(cycles) + type3_extensions(cycles)
+
+
+def combination_extensions(cycles: Cycles) -> List[Tuple[str, Cycles]]:
+    """Combinations.extend (:64-94): every bad small component, labels shifted
+    by the number of cycles (float32 sums), inserted before every position."""
+    sig = signature(cycles)
+    k = F32(len(cycles))
+    out = []
+    for comp, info in BAD_SMALL_COMPONENTS:
+        shifted = (signature(comp) + k).astype(F32)
+        for i in range(len(sig)):
+            out.append((f"pos={i} add {info}", of_signature(np.concatenate([sig[:i], shifted, sig[i:]]))))
+    return out
+
+
+# ---------------------------------------------------------------- exploration
+SEARCH_BATCH_MAX = 65536      # configurations per batched search (16-bit index in the dedup tag)
+
+@dataclass
+class Level:
+    depth: int
+    configurations: int          # distinct configurations of the level
+    new_canonical: int           # canonical forms searched at this level
+    found: int                   # of those, with a sorting
+    extended: int                # configurations with no sorting (extended unless the level is the last)
+    max_three_norm: int
+    seconds: Dict[str, float] = field(default_factory=dict)
+
+
+@dataclass
+class Exploration:
+    levels: List[Level]
+    verdicts: Dict[Tuple[int, ...], bool]             # canonical one-line spi -> sorting found
+    sortings: Dict[Tuple[int, ...], List[Tuple[int, int, int]]]
+    truncated: bool
+
+
+def explore(roots: Sequence[Cycles], extend: Callable[[Cycles], List[Tuple[str, Cycles]]], max_levels: int,
+            max_ratio: float = 1.375, max_depth: int = 11, slot: int = 0, max_configurations: int = 65536,
+            log: Optional[Callable[[str], None]] = None) -> Exploration:
+    """The SortOrExtend tree from roots, level by level, to max_levels levels.
+
+    A configuration already visited is not visited again (its extensions
+    would be the same); a canonical form is searched once, the first time it
+    is reached (the single-flight coordinator and the sorting / bad-case files
+    of the reference); every configuration whose canonical form has no
+    sorting is extended.  A level with more than max_configurations distinct
+    configurations stops the exploration (truncated=True)."""
+    from . import canonical as K
+    from . import proof_driver as D
+
+    seen: Set[Tuple[int, ...]] = set()
+    verdicts: Dict[Tuple[int, ...], bool] = {}
+    sortings: Dict[Tuple[int, ...], List[Tuple[int, int, int]]] = {}
+    levels: List[Level] = []
+    frontier = [list(map(list, r)) for r in roots]
+    truncated = False
+    for depth in range(max_levels):
+        t0 = time.perf_counter()
+        uniq, keys = [], []
+        for c in frontier:
+            key = tuple(D.one_line(c))
+            if key not in seen:
+                seen.add(key)
+                uniq.append(c)
+                keys.append(key)
+        if not uniq:
+            break
+        if len(uniq) > max_configurations:
+            truncated = True
+            break
+        t1 = time.perf_counter()
+        canon = K.canonical_batch([list(k) for k in keys])
+        ckeys = [tuple(r.spi) for r in canon]
+        t2 = time.perf_counter()
+        todo = sorted({k for k in ckeys if k not in verdicts})
+        found = 0
+        for lo in range(0, len(todo), SEARCH_BATCH_MAX):
+            part = todo[lo:lo + SEARCH_BATCH_MAX]
+            res = D.sort_batch(slot, [K.cycles_of(list(k)) for k in part], max_ratio, max_depth)
+            for q, k in enumerate(part):
+                verdicts[k] = bool(res.found[q])
+                if res.found[q]:
+                    sortings[k] = res.moves[q]
+                    found += 1
+        t3 = time.perf_counter()
+        nxt = []
+        extended = 0
+        last = depth + 1 == max_levels
+        tick = time.perf_counter()
+        for c, k in zip(uniq, ckeys):
+            if not verdicts[k]:
+                extended += 1
+                if not last:
+                    nxt.extend(cfg for _, cfg in extend(c))
+                    if log and time.perf_counter() - tick > 30:
+                        tick = time.perf_counter()
+                        log(f"level {depth}: extended {extended} configurations into {len(nxt)}")
+        t4 = time.perf_counter()
+        lv = Level(depth, len(uniq), len(todo), found, extended, max(D.three_norm(c) for c in uniq),
+                   {"dedup": t1 - t0, "canonical": t2 - t1, "search": t3 - t2, "extend": t4 - t3})
+        levels.append(lv)
+        if log:
+            log(f"level {depth}: {lv.configurations} configurations, {lv.new_canonical} new canonical, "
+                f"{lv.found} found, {lv.extended} extended, 3-norm <= {lv.max_three_norm}, "
+                + ", ".join(f"{k} {v:.2f}s" for k, v in lv.seconds.items()))
+        frontier = nxt
+    return Exploration(levels, verdicts, sortings, truncated)
+
+
+def main(argv=None):
+    """python -m tdp1375_amd.extensions --tree dfs --levels 3"""
+    import argparse
+    import json
+    import sys
+
+    from . import gpu_sorting_search as G
+    ap = argparse.ArgumentParser(description="level-by-level SortOrExtend exploration on the GPU")
+    ap.add_argument("--tree", choices=("dfs", "comb"), default="dfs")
+    ap.add_argument("--levels", type=int, default=3)
+    ap.add_argument("--max-ratio", type=float, default=1.375)
+    ap.add_argument("--max-depth", type=int, default=11)
+    ap.add_argument("--max-configurations", type=int, default=1 << 21)
+    ap.add_argument("--queue-mb", type=int, default=16384)
+    ap.add_argument("--dedup-log2", type=int, default=26)
+    args = ap.parse_args(argv)
+    roots = DFS_ROOTS if args.tree == "dfs" else [c for c, _ in BAD_SMALL_COMPONENTS]
+    ext = dfs_extensions if args.tree == "dfs" else combination_extensions
+    G.init_slots(1, 1, args.queue_mb << 20, 1 << args.dedup_log2)
+    try:
+        t0 = time.perf_counter()
+        res = explore(roots, ext, args.levels, args.max_ratio, args.max_depth,
+                      max_configurations=args.max_configurations,
+                      log=lambda m: print(m, file=sys.stderr, flush=True))
+        wall = time.perf_counter() - t0
+    finally:
+        G.destroy_slots()
+    print(json.dumps({"tree": args.tree, "levels": [vars(l) for l in res.levels], "truncated": res.truncated,
+                      "canonical_searched": len(res.verdicts),
+                      "bad": sum(1 for v in res.verdicts.values() if not v), "seconds": wall}), flush=True)
+
+
+if __name__ == "__main__":
+    main()
