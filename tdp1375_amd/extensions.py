@@ -398,6 +398,7 @@ class Level:
     found: int                   # of those, with a sorting
     extended: int                # configurations with no sorting (extended unless the level is the last)
     max_three_norm: int
+    max_bad_three_norm: int      # largest 3-norm among the configurations with no sorting (-1: none)
     seconds: Dict[str, float] = field(default_factory=dict)
 
 
@@ -471,12 +472,14 @@ def explore(roots: Sequence[Cycles], extend: Callable[[Cycles], List[Tuple[str, 
                         tick = time.perf_counter()
                         log(f"level {depth}: extended {extended} configurations into {len(nxt)}")
         t4 = time.perf_counter()
+        bad_norms = [D.three_norm(c) for c, k in zip(uniq, ckeys) if not verdicts[k]]
         lv = Level(depth, len(uniq), len(todo), found, extended, max(D.three_norm(c) for c in uniq),
-                   {"dedup": t1 - t0, "canonical": t2 - t1, "search": t3 - t2, "extend": t4 - t3})
+                   max(bad_norms, default=-1), {"dedup": t1 - t0, "canonical": t2 - t1, "search": t3 - t2, "extend": t4 - t3})
         levels.append(lv)
         if log:
             log(f"level {depth}: {lv.configurations} configurations, {lv.new_canonical} new canonical, "
-                f"{lv.found} found, {lv.extended} extended, 3-norm <= {lv.max_three_norm}, "
+                f"{lv.found} found, {lv.extended} without a sorting (3-norm <= {lv.max_bad_three_norm}), "
+                f"3-norm <= {lv.max_three_norm}, "
                 + ", ".join(f"{k} {v:.2f}s" for k, v in lv.seconds.items()))
         frontier = nxt
     return Exploration(levels, verdicts, sortings, truncated)

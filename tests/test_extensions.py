@@ -286,3 +286,14 @@ def test_dfs_tree_three_levels_on_the_gpu(slots):
             positions.append((i, j, l))
             cur = cur[:i] + cur[j:l] + cur[i:j] + cur[l:]
         assert_witness(pi, list(k), PD.odd_cycles(cyc), min_rate, 11, positions)
+
+
+@pytest.mark.gpu
+def test_combination_tree_runs_to_completion_on_the_gpu(slots):
+    """index.html:63: no combination of bad small components with 3-norm
+    greater than 7 was extended, and the search over combinations ends."""
+    res = E.explore([c for c, _ in E.BAD_SMALL_COMPONENTS], E.combination_extensions, 8, 1.375, 11, slot=slots[0])
+    assert not res.truncated
+    assert res.levels[-1].extended == 0                  # the last level has a sorting for everything
+    assert max(l.max_bad_three_norm for l in res.levels) == 7
+    assert res.levels[0].extended == 6                   # every bad small component lacks a sorting
