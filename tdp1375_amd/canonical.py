@@ -99,6 +99,33 @@ def canonical_batch(spis: Sequence[Sequence[int]], pis: Optional[Sequence[Sequen
             for c in range(count)]
 
 
+def canonical_rows(spis: np.ndarray, ns: np.ndarray):
+    """canonical_batch on pre-packed rows: spis (count, 64) uint8 one-line
+    spi (entries past ns[c] ignored), pi the identity.  Returns (signatures
+    float32 (count, 64), hashes int32, indices int32, canonical spis uint8
+    (count, 64), entries past ns[c] zero)."""
+    count = len(ns)
+    S = np.ascontiguousarray(spis, np.uint8)
+    if S.shape != (count, MAX_N):
+        raise ValueError("spis must be (count, 64) uint8")
+    nsa = np.ascontiguousarray(ns, np.int32)
+    P = np.ascontiguousarray(np.broadcast_to(np.arange(MAX_N, dtype=np.uint8), (count, MAX_N)))
+    sig = np.zeros((count, MAX_N), np.float32)
+    h = np.zeros(count, np.int32)
+    idx = np.zeros(count, np.int32)
+    out = np.zeros((count, MAX_N), np.uint8)
+    if count == 0:
+        return sig, h, idx, out
+    u8p = ctypes.POINTER(ctypes.c_uint8)
+    i32p = ctypes.POINTER(ctypes.c_int32)
+    _lib.check(_lib.load().tdp_canonical_batch(count, S.ctypes.data_as(u8p), P.ctypes.data_as(u8p),
+                                               nsa.ctypes.data_as(i32p),
+                                               sig.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                                               h.ctypes.data_as(i32p), idx.ctypes.data_as(i32p),
+                                               out.ctypes.data_as(u8p)))
+    return sig, h, idx, out
+
+
 def canonical(spi: str) -> Canonical:
     """Canonical form of one configuration given as a cycle string over the
     canonical pi (new Configuration(spi).getCanonical())."""

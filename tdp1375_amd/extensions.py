@@ -387,6 +387,176 @@ def combination_extensions(cycles: Cycles) -> List[Tuple[str, Cycles]]:
     return out
 
 
+# ---------------------------------------------------------------- one-line rows
+# The same extensions as one-line spi rows (pi the identity), built for all
+# insertion positions of a parent at once; explore works on these.  A result
+# is (rows, ns): rows (K, 64) uint8 padded with 255, ns (K,) the sizes; row r
+# is the permutation of the r-th configuration the matching (info, cycles)
+# generator above returns.
+ROW_WIDTH = 64
+PAD = 255
+
+
+def _pad(parts: List[np.ndarray]) -> Tuple[np.ndarray, np.ndarray]:
+    parts = [q for q in parts if len(q)]
+    K = sum(len(q) for q in parts)
+    rows = np.full((K, ROW_WIDTH), PAD, np.uint8)
+    ns = np.empty(K, np.int32)
+    at = 0
+    for q in parts:
+        if q.shape[1] > ROW_WIDTH:
+            raise ValueError("an extension exceeds 64 symbols")
+        rows[at:at + len(q), :q.shape[1]] = q
+        ns[at:at + len(q)] = q.shape[1]
+        at += len(q)
+    return rows, ns
+
+def cycles_of_row(row: Sequence[int]) -> Cycles:
+    """Cycles of a one-line spi, each from its smallest symbol, in order of
+    their smallest symbols (the order of their labels in the signature)."""
+    p = [int(x) for x in row]
+    seen = [False] * len(p)
+    out = []
+    for s in range(len(p)):
+        if not seen[s]:
+            c, x = [], s
+            while not seen[x]:
+                seen[x] = True
+                c.append(x)
+                x = p[x]
+            out.append(c)
+    return out
+
+
+def _moved(p: np.ndarray, M: np.ndarray, extra: int) -> np.ndarray:
+    """Rows of n + extra symbols with every parent symbol s moved to M[r, s]
+    (and its image to M[r, p[s]]); the inserted symbols are left to the
+    caller."""
+    K, n = M.shape
+    out = np.empty((K, n + extra), np.int64)
+    out[np.arange(K)[:, None], M] = M[:, p]
+    return out
+
+
+def _new_3cycle_rows(p: np.ndarray, T: np.ndarray) -> np.ndarray:
+    n = len(p)
+    if len(T) == 0:
+        return np.empty((0, n + 3), np.int64)
+    ar = np.arange(n)[None, :]
+    M = ar + (ar >= T[:, 0:1]) + (ar >= T[:, 1:2]) + (ar >= T[:, 2:3])
+    out = _moved(p, M, 3)
+    r = np.arange(len(T))
+    c0, c1, c2 = T[:, 2] + 2, T[:, 1] + 1, T[:, 0]
+    out[r, c0] = c1
+    out[r, c1] = c2
+    out[r, c2] = c0
+    return out
+
+
+def _oriented(cycle: List[int], n: int) -> bool:
+    inv = [(n - x) % n for x in cycle]
+    return sum(1 for t in range(len(inv)) if inv[t] > inv[(t + 1) % len(inv)]) > 1
+
+
+def dfs_extension_rows(row: Sequence[int]) -> Tuple[np.ndarray, np.ndarray]:
+    """dfs_extensions as one-line rows (type 1, then type 2, then type 3)."""
+    p = np.asarray(row, np.int64)
+    n = len(p)
+    cycles = cycles_of_row(p)
+    gates = sorted(open_gates(cycles, n))
+    bi, ci = np.triu_indices(n)
+    parts = []
+    # type 1: gate i, b <= c, not all equal, positions sorted
+    if gates:
+        I = np.repeat(np.asarray(gates), len(bi))
+        B = np.tile(bi, len(gates))
+        C = np.tile(ci, len(gates))
+        keep = ~((I == B) & (B == C))
+        parts.append(_new_3cycle_rows(p, np.sort(np.stack([I, B, C], 1)[keep], axis=1)))
+    else:
+        # type 2: a full configuration, a <= b <= c, not all equal
+        T = np.asarray([(a, b, c) for a in range(n) for b in range(a, n) for c in range(b, n)
+                        if not (a == b == c)], np.int64).reshape(-1, 3)
+        parts.append(_new_3cycle_rows(p, T))
+    # type 3
+    A, Bp = np.triu_indices(n)
+    K = len(A)
+    r = np.arange(K)
+    ar = np.arange(n)[None, :]
+    M = ar + (ar >= A[:, None]) + (ar >= Bp[:, None])
+    moved = _moved(p, M, 2)
+    cid = np.empty(n, np.int64)
+    for k, c in enumerate(cycles):
+        cid[c] = k
+    cid_moved = np.empty((K, n + 2), np.int64)
+    cid_moved[r[:, None], M] = cid[None, :]
+    for k, c in enumerate(cycles):                # label k + 1: cycles in order of their smallest symbols
+        if _oriented(c, n):
+            continue
+        pos = np.asarray(sorted(c))
+        L = len(pos)
+        stays = (np.searchsorted(pos, A) % L) == (np.searchsorted(pos, Bp) % L)
+        keep = np.zeros(K, bool)
+        rows = moved.copy()
+        if stays.any():
+            srt = -np.sort(-np.concatenate([M[:, pos], A[:, None], Bp[:, None] + 1], axis=1), axis=1)
+            grown = rows[stays]
+            g = srt[stays]
+            gr = np.arange(len(g))[:, None]
+            grown[gr, g] = np.roll(g, -1, axis=1)
+            cm = cid_moved[stays].copy()
+            cm[np.arange(len(g)), A[stays]] = k
+            cm[np.arange(len(g)), Bp[stays] + 1] = k
+            rows[stays] = grown
+            keep[stays] = _open_gate_counts(grown, cm) <= 2
+        if L == 3:
+            m = int(np.argmax(c))
+            top = c[m]
+            rot = [top, int(p[top]), int(p[p[top]])]
+            turn = ~stays
+            q0, q1, q2 = (M[turn][:, x] for x in rot)
+            a, b1 = A[turn], Bp[turn] + 1
+            d = [(x - q0) % (n + 2) for x in (q2, a, q1, b1)]
+            ok = (d[0] < d[1]) & (d[1] < d[2]) & (d[2] < d[3])
+            tr = rows[turn]
+            t = np.arange(len(tr))
+            tr[t, q0] = q1
+            tr[t, q1] = q2
+            tr[t, q2] = b1
+            tr[t, b1] = a
+            tr[t, a] = q0
+            rows[turn] = tr
+            keep[turn] = ok
+        parts.append(rows[keep])
+    return _pad(parts)
+
+
+def combination_extension_rows(row: Sequence[int]) -> Tuple[np.ndarray, np.ndarray]:
+    """combination_extensions as one-line rows."""
+    p = np.asarray(row, np.int64)
+    n = len(p)
+    ar = np.arange(n)[None, :]
+    I = np.arange(n)[:, None]
+    parts = []
+    for comp, _ in BAD_SMALL_COMPONENTS:
+        pb = np.asarray(one_line_of(comp), np.int64)
+        nb = len(pb)
+        M = ar + nb * (ar >= I)
+        out = _moved(p, M, nb)
+        out[np.arange(n)[:, None], I + np.arange(nb)[None, :]] = I + pb[None, :]
+        parts.append(out)
+    return _pad(parts)
+
+
+def one_line_of(cycles: Cycles) -> List[int]:
+    n = size(cycles)
+    p = list(range(n))
+    for c in cycles:
+        for t in range(len(c)):
+            p[c[t]] = c[t + 1 - len(c)]
+    return p
+
+
 # ---------------------------------------------------------------- exploration
 SEARCH_BATCH_MAX = 65536      # configurations per batched search (16-bit index in the dedup tag)
 
@@ -410,79 +580,92 @@ class Exploration:
     truncated: bool
 
 
-def explore(roots: Sequence[Cycles], extend: Callable[[Cycles], List[Tuple[str, Cycles]]], max_levels: int,
-            max_ratio: float = 1.375, max_depth: int = 11, slot: int = 0, max_configurations: int = 65536,
-            log: Optional[Callable[[str], None]] = None) -> Exploration:
-    """The SortOrExtend tree from roots, level by level, to max_levels levels.
+def _keys(rows: np.ndarray) -> List[bytes]:
+    return np.ascontiguousarray(rows).view(np.dtype((np.void, ROW_WIDTH))).ravel().tolist()
+
+
+def explore(roots: Sequence[Cycles], extend_rows: Callable[[Sequence[int]], Tuple[np.ndarray, np.ndarray]],
+            max_levels: int, max_ratio: float = 1.375, max_depth: int = 11, slot: int = 0,
+            max_configurations: int = 1 << 21, log: Optional[Callable[[str], None]] = None) -> Exploration:
+    """The SortOrExtend tree from roots, level by level, to max_levels levels,
+    with extend_rows (dfs_extension_rows or combination_extension_rows).
 
     A configuration already visited is not visited again (its extensions
     would be the same); a canonical form is searched once, the first time it
     is reached (the single-flight coordinator and the sorting / bad-case files
     of the reference); every configuration whose canonical form has no
     sorting is extended.  A level with more than max_configurations distinct
-    configurations stops the exploration (truncated=True)."""
+    configurations stops the exploration (truncated=True).  verdicts and
+    sortings are keyed by the canonical one-line spi as a tuple."""
     from . import canonical as K
     from . import proof_driver as D
 
-    seen: Set[Tuple[int, ...]] = set()
-    verdicts: Dict[Tuple[int, ...], bool] = {}
+    seen: Set[bytes] = set()
+    verdicts: Dict[bytes, bool] = {}
+    norms: Dict[bytes, int] = {}
     sortings: Dict[Tuple[int, ...], List[Tuple[int, int, int]]] = {}
     levels: List[Level] = []
-    frontier = [list(map(list, r)) for r in roots]
+    rows, ns = _pad([np.asarray([one_line_of(r)], np.int64) for r in roots])
     truncated = False
     for depth in range(max_levels):
         t0 = time.perf_counter()
-        uniq, keys = [], []
-        for c in frontier:
-            key = tuple(D.one_line(c))
-            if key not in seen:
-                seen.add(key)
-                uniq.append(c)
-                keys.append(key)
-        if not uniq:
+        view = np.ascontiguousarray(rows).view(np.dtype((np.void, ROW_WIDTH))).ravel()
+        keys = view.tolist()
+        first = np.sort(np.unique(view, return_index=True)[1])
+        fresh = [int(i) for i in first if keys[i] not in seen]
+        if not fresh:
             break
-        if len(uniq) > max_configurations:
+        if len(fresh) > max_configurations:
             truncated = True
             break
+        seen.update(keys[i] for i in fresh)
+        rows, ns = rows[fresh], ns[fresh]
         t1 = time.perf_counter()
-        canon = K.canonical_batch([list(k) for k in keys])
-        ckeys = [tuple(r.spi) for r in canon]
+        _, _, _, canon = K.canonical_rows(rows, ns)
+        canon[np.arange(ROW_WIDTH)[None, :] >= ns[:, None]] = PAD
+        ckeys = _keys(canon)
         t2 = time.perf_counter()
         todo = sorted({k for k in ckeys if k not in verdicts})
         found = 0
         for lo in range(0, len(todo), SEARCH_BATCH_MAX):
             part = todo[lo:lo + SEARCH_BATCH_MAX]
-            res = D.sort_batch(slot, [K.cycles_of(list(k)) for k in part], max_ratio, max_depth)
+            perms = [[b for b in k if b != PAD] for k in part]
+            cfgs = [cycles_of_row(q) for q in perms]
+            res = D.sort_batch(slot, cfgs, max_ratio, max_depth)
             for q, k in enumerate(part):
                 verdicts[k] = bool(res.found[q])
+                norms[k] = D.three_norm(cfgs[q])
                 if res.found[q]:
-                    sortings[k] = res.moves[q]
+                    sortings[tuple(perms[q])] = res.moves[q]
                     found += 1
         t3 = time.perf_counter()
-        nxt = []
-        extended = 0
+        bad = [i for i, k in enumerate(ckeys) if not verdicts[k]]
         last = depth + 1 == max_levels
+        parts_r, parts_n = [], []
         tick = time.perf_counter()
-        for c, k in zip(uniq, ckeys):
-            if not verdicts[k]:
-                extended += 1
-                if not last:
-                    nxt.extend(cfg for _, cfg in extend(c))
-                    if log and time.perf_counter() - tick > 30:
-                        tick = time.perf_counter()
-                        log(f"level {depth}: extended {extended} configurations into {len(nxt)}")
+        if not last:
+            for t, i in enumerate(bad):
+                r_, n_ = extend_rows(rows[i, :ns[i]])
+                parts_r.append(r_)
+                parts_n.append(n_)
+                if log and time.perf_counter() - tick > 30:
+                    tick = time.perf_counter()
+                    log(f"level {depth}: extended {t + 1} of {len(bad)} configurations")
         t4 = time.perf_counter()
-        bad_norms = [D.three_norm(c) for c, k in zip(uniq, ckeys) if not verdicts[k]]
-        lv = Level(depth, len(uniq), len(todo), found, extended, max(D.three_norm(c) for c in uniq),
-                   max(bad_norms, default=-1), {"dedup": t1 - t0, "canonical": t2 - t1, "search": t3 - t2, "extend": t4 - t3})
+        lv = Level(depth, len(fresh), len(todo), found, len(bad), max(norms[k] for k in set(ckeys)),
+                   max((norms[ckeys[i]] for i in bad), default=-1),
+                   {"dedup": t1 - t0, "canonical": t2 - t1, "search": t3 - t2, "extend": t4 - t3})
         levels.append(lv)
         if log:
             log(f"level {depth}: {lv.configurations} configurations, {lv.new_canonical} new canonical, "
                 f"{lv.found} found, {lv.extended} without a sorting (3-norm <= {lv.max_bad_three_norm}), "
                 f"3-norm <= {lv.max_three_norm}, "
                 + ", ".join(f"{k} {v:.2f}s" for k, v in lv.seconds.items()))
-        frontier = nxt
-    return Exploration(levels, verdicts, sortings, truncated)
+        if last or not parts_r:
+            break
+        rows, ns = np.concatenate(parts_r), np.concatenate(parts_n)
+    return Exploration(levels, {tuple(b for b in k if b != PAD): v for k, v in verdicts.items()}, sortings,
+                       truncated)
 
 
 def main(argv=None):
@@ -502,7 +685,7 @@ def main(argv=None):
     ap.add_argument("--dedup-log2", type=int, default=26)
     args = ap.parse_args(argv)
     roots = DFS_ROOTS if args.tree == "dfs" else [c for c, _ in BAD_SMALL_COMPONENTS]
-    ext = dfs_extensions if args.tree == "dfs" else combination_extensions
+    ext = dfs_extension_rows if args.tree == "dfs" else combination_extension_rows
     G.init_slots(1, 1, args.queue_mb << 20, 1 << args.dedup_log2)
     try:
         t0 = time.perf_counter()

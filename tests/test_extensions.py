@@ -148,6 +148,12 @@ def test_generators_equal_the_oracle():
         assert E.open_gates(c) == R.open_gates(c), c
         for name in ("type1_extensions", "type2_extensions", "type3_extensions", "combination_extensions"):
             assert getattr(E, name)(c) == getattr(R, name)(c), (name, c)
+        # the one-line rows explore works on: the same configurations in the same order
+        for api, rows_of in ((R.dfs_extensions, E.dfs_extension_rows),
+                             (R.combination_extensions, E.combination_extension_rows)):
+            rows, ns = rows_of(E.one_line_of(c))
+            assert [list(map(int, rows[i, :ns[i]])) for i in range(len(ns))] == \
+                [perm(x, E.size(x)) for _, x in api(c)], (api.__name__, c)
 
 
 def test_extension_counts_follow_the_closed_forms():
@@ -239,28 +245,29 @@ def test_exploration_searches_each_canonical_form_once_and_extends_only_bad_ones
     from tdp1375_amd import canonical as K
     searched = []
 
-    def canon(spis, pis=None):
-        out = []
-        for p in spis:
-            sig, h, idx = C.canonical(C.cycles_of(p), list(range(len(p))))
-            out.append(K.Canonical(sig, h, idx, perm(C.of_signature(sig), len(p))))
-        return out
+    def canon(rows, ns):
+        out = np.zeros((len(ns), 64), np.uint8)
+        for i, n in enumerate(ns):
+            cyc = C.cycles_of([int(x) for x in rows[i, :n]])
+            sig, _, _ = C.canonical(cyc, list(range(n)))
+            out[i, :n] = perm(C.of_signature(sig), n)
+        return None, None, None, out
 
     def sort_batch(slot, configs, ratio, depth, flags=0):
         searched.extend(tuple(perm(c, E.size(c))) for c in configs)
         found = np.array([any(float(v) % 1 for v in E.signature(c)) for c in configs])
         return PD.LadderResult(found, np.full(len(configs), 4), [[(0, 1, 2)] if f else None for f in found], 1)
 
-    monkeypatch.setattr(K, "canonical_batch", canon)
+    monkeypatch.setattr(K, "canonical_rows", canon)
     monkeypatch.setattr(PD, "sort_batch", sort_batch)
-    res = E.explore([E.INTERLEAVING_PAIR, E.INTERLEAVING_PAIR], E.dfs_extensions, 2)
+    res = E.explore([E.INTERLEAVING_PAIR, E.INTERLEAVING_PAIR], E.dfs_extension_rows, 2)
     assert len(searched) == len(set(searched)) == len(res.verdicts)
     assert res.levels[0].configurations == 1 and res.levels[0].extended == 1
     lv = res.levels[1]
     kids = {tuple(perm(x, E.size(x))) for _, x in E.dfs_extensions(E.INTERLEAVING_PAIR)}
     assert lv.configurations == len(kids)
     assert lv.extended == sum(1 for k in kids if not any(float(v) % 1 for v in E.signature(C.cycles_of(list(k)))))
-    res = E.explore([E.INTERLEAVING_PAIR], E.dfs_extensions, 3, max_configurations=10)
+    res = E.explore([E.INTERLEAVING_PAIR], E.dfs_extension_rows, 3, max_configurations=10)
     assert res.truncated and len(res.levels) == 1
 
 
@@ -269,7 +276,7 @@ def test_exploration_searches_each_canonical_form_once_and_extends_only_bad_ones
 def test_dfs_tree_three_levels_on_the_gpu(slots):
     from helpers import assert_witness
     g = fixture()
-    res = E.explore(E.DFS_ROOTS, E.dfs_extensions, 3, 1.375, 11, slot=slots[0])
+    res = E.explore(E.DFS_ROOTS, E.dfs_extension_rows, 3, 1.375, 11, slot=slots[0])
     assert [l.configurations for l in res.levels][:2] == [3, 123]
     min_rate = float(np.float32(2.0) / np.float32(1.375))
     for r in E.DFS_ROOTS:                      # the roots do not allow 11/8 (index.html:52-56)
@@ -292,7 +299,8 @@ def test_dfs_tree_three_levels_on_the_gpu(slots):
 def test_combination_tree_runs_to_completion_on_the_gpu(slots):
     """index.html:63: no combination of bad small components with 3-norm
     greater than 7 was extended, and the search over combinations ends."""
-    res = E.explore([c for c, _ in E.BAD_SMALL_COMPONENTS], E.combination_extensions, 8, 1.375, 11, slot=slots[0])
+    res = E.explore([c for c, _ in E.BAD_SMALL_COMPONENTS], E.combination_extension_rows, 8, 1.375, 11,
+                    slot=slots[0])
     assert not res.truncated
     assert res.levels[-1].extended == 0                  # the last level has a sorting for everything
     assert max(l.max_bad_three_norm for l in res.levels) == 7
