@@ -96,9 +96,10 @@ def test_positions_to_symbols_replay():
 
 def test_plus2_children_stay_in_one_cycle_or_two_even_cycles():
     """Premise of the kernel's cycle-local enumeration: a child with two more
-    odd cycles moves three symbols of one cycle, or two symbols of one even
-    cycle and one of another even cycle (checked with the oracle's O(n)
-    recount of every child, on nodes with and without even cycles)."""
+    odd cycles moves three symbols of one oriented cycle (two or more ascents
+    of the pi-positions along spi), or two symbols of one even cycle and one
+    of another even cycle (checked with the oracle's O(n) recount of every
+    child, on nodes with and without even cycles)."""
     rng = np.random.default_rng(5)
     for it in range(80):
         n = int(rng.integers(5, 40))
@@ -122,6 +123,8 @@ def test_plus2_children_stay_in_one_cycle_or_two_even_cycles():
         cyc_of = {s: c for c, cyc in enumerate(cycles) for s in cyc}
         even = {c for c, cyc in enumerate(cycles) if len(cyc) % 2 == 0}
         E = sum(len(cycles[c]) for c in even)
+        pos = {x: q for q, x in enumerate(pi)}
+        ascents = [sum(1 for t in range(len(c)) if pos[c[(t + 1) % len(c)]] > pos[c[t]]) for c in cycles]
         kids = oracle.children_even(pi, spi)
         t = 0
         plus2 = 0
@@ -132,7 +135,8 @@ def test_plus2_children_stay_in_one_cycle_or_two_even_cycles():
                         plus2 += 1
                         cs = sorted({cyc_of[pi[i]], cyc_of[pi[j]], cyc_of[pi[k]]})
                         assert len(cs) == 1 or (len(cs) == 2 and set(cs) <= even), (it, i, j, k)
+                        assert len(cs) == 2 or ascents[cs[0]] >= 2, (it, i, j, k)
                     t += 1
-        bound = sum(L * (L - 1) * (L - 2) // 6 for L in lengths)
+        bound = sum(len(c) * (len(c) - 1) * (len(c) - 2) // 6 for c, a in zip(cycles, ascents) if a >= 2)
         bound += sum(len(cycles[c]) * (len(cycles[c]) - 1) // 2 * (E - len(cycles[c])) for c in even)
         assert plus2 <= bound
