@@ -629,14 +629,16 @@ def explore(roots: Sequence[Cycles], extend_rows: Callable[[Sequence[int]], Tupl
         found = 0
         for lo in range(0, len(todo), SEARCH_BATCH_MAX):
             part = todo[lo:lo + SEARCH_BATCH_MAX]
-            perms = [[b for b in k if b != PAD] for k in part]
-            cfgs = [cycles_of_row(q) for q in perms]
-            res = D.sort_batch(slot, cfgs, max_ratio, max_depth)
+            R = np.frombuffer(b"".join(part), np.uint8).reshape(len(part), ROW_WIDTH)
+            N = (R != PAD).sum(axis=1).astype(np.int32)
+            e0 = D.odd_cycle_counts(R, N)
+            three = (N - e0) // 2
+            res = D.sort_rows(slot, R, N, max_ratio, max_depth, e0=e0, norms=three)
             for q, k in enumerate(part):
                 verdicts[k] = bool(res.found[q])
-                norms[k] = D.three_norm(cfgs[q])
+                norms[k] = int(three[q])
                 if res.found[q]:
-                    sortings[tuple(perms[q])] = res.moves[q]
+                    sortings[tuple(R[q, :N[q]].tolist())] = res.moves[q]
                     found += 1
         t3 = time.perf_counter()
         bad = [i for i, k in enumerate(ckeys) if not verdicts[k]]

@@ -142,6 +142,22 @@ def search_batch(slot: int, pis: Sequence[Sequence[int]], spis: Sequence[Sequenc
         ns[c] = n
         P[c, :n] = np.asarray(pis[c], np.uint8)
         S[c, :n] = np.asarray(spis[c][:n], np.uint8)
+    return search_batch_rows(slot, P, S, ns, initial_even_cycles, min_rates, max_moves, full_sorting, flags)
+
+
+_CASE_DTYPE = np.dtype(_lib.CaseResult)
+
+
+def search_batch_rows(slot: int, P: np.ndarray, S: np.ndarray, ns, initial_even_cycles, min_rates, max_moves,
+                      full_sorting=None, flags: int = 0, want_moves=None) -> BatchResult:
+    """search_batch on pre-packed rows: P and S (count, 64) uint8 (pi and spi
+    of configuration c in their first ns[c] entries).  Moves are returned as
+    position triples for the configurations want_moves selects (a boolean
+    mask; default all), an empty list for the others."""
+    count = len(ns)
+    P = np.ascontiguousarray(P, np.uint8)
+    S = np.ascontiguousarray(S, np.uint8)
+    nsa = np.ascontiguousarray(ns, np.int32)
     e0 = np.ascontiguousarray(initial_even_cycles, np.int32)
     mr = np.ascontiguousarray(np.broadcast_to(np.float32(min_rates) if np.isscalar(min_rates) else
                                               np.asarray(min_rates, np.float32), (count,)), np.float32)
@@ -152,17 +168,18 @@ def search_batch(slot: int, pis: Sequence[Sequence[int]], spis: Sequence[Sequenc
     u8p = ctypes.POINTER(ctypes.c_uint8)
     i32p = ctypes.POINTER(ctypes.c_int32)
     st = _lib.load().tdp_search_batch(slot, count, P.ctypes.data_as(u8p), S.ctypes.data_as(u8p),
-                                      ns.ctypes.data_as(i32p), e0.ctypes.data_as(i32p),
+                                      nsa.ctypes.data_as(i32p), e0.ctypes.data_as(i32p),
                                       mr.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), md.ctypes.data_as(i32p),
                                       fs.ctypes.data_as(i32p), flags, out)
     _lib.check(st)
-    found = np.array([bool(out[c].found) for c in range(count)])
-    depth = np.array([int(out[c].depth) for c in range(count)], np.int32)
-    moves = [[(out[c].moves[3 * m], out[c].moves[3 * m + 1], out[c].moves[3 * m + 2])
-              for m in range(max(out[c].depth, 0))]
+    arr = np.frombuffer(out, dtype=_CASE_DTYPE, count=count)
+    found = arr["found"] != 0
+    depth = arr["depth"].astype(np.int32)
+    mv = arr["moves"].reshape(count, -1, 3)
+    sel = np.ones(count, bool) if want_moves is None else np.asarray(want_moves, bool)
+    moves = [[tuple(m) for m in mv[c, :max(int(depth[c]), 0)].tolist()] if sel[c] else []
              for c in range(count)]
-    counters = {name: np.array([getattr(out[c].counters, name) for c in range(count)], np.uint64)
-                for name, _ in _lib.Counters._fields_}
+    counters = {name: arr["counters"][name].astype(np.uint64) for name, _ in _lib.Counters._fields_}
     return BatchResult(found, depth, moves, counters)
 
 

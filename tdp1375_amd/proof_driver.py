@@ -36,7 +36,7 @@ from typing import Callable, Dict, Iterable, List, Optional, Sequence, Tuple
 
 import numpy as np
 
-from .gpu_sorting_search import GPUSortingSearch, positions_to_symbols, search_batch
+from .gpu_sorting_search import GPUSortingSearch, search_batch_rows
 
 Cycles = Sequence[Sequence[int]]
 Move = Tuple[int, int, int]
@@ -227,28 +227,86 @@ def sort_batch(slot: int, configs: Sequence[Cycles], max_ratio: float, max_depth
                flags: int = 0) -> LadderResult:
     """The ladder of SortOrExtend.searchForSorting for a whole list of
     configurations, one batched launch per rung."""
-    min_rate = float(np.float32(2.0) / np.float32(max_ratio))
     count = len(configs)
-    spis = [one_line(c) for c in configs]
-    e0s = [odd_cycles(c) for c in configs]
-    ladders = [depth_ladder(upper_bound(c, min_rate), max_depth) for c in configs]
+    rows = np.zeros((count, 64), np.uint8)
+    ns = np.zeros(count, np.int32)
+    for c, cfg in enumerate(configs):
+        spi = one_line(cfg)
+        ns[c] = len(spi)
+        rows[c, :len(spi)] = spi
+    return sort_rows(slot, rows, ns, max_ratio, max_depth, e0=[odd_cycles(c) for c in configs],
+                     norms=[three_norm(c) for c in configs], flags=flags)
+
+
+def odd_cycle_counts(rows: np.ndarray, ns) -> np.ndarray:
+    """Odd-length cycles (fixed points included) of every one-line row's
+    first ns[c] entries, all rows at once: each symbol's cycle minimum by
+    pointer doubling, cycle lengths by counting symbols per minimum."""
+    rows = np.asarray(rows)
+    ns = np.asarray(ns)
+    K, W = rows.shape
+    x = np.arange(W)
+    inside = x[None, :] < ns[:, None]
+    q = np.where(inside, rows.astype(np.int64), x[None, :])
+    m = np.broadcast_to(x, (K, W)).copy()
+    for _ in range(6):
+        m = np.minimum(m, np.take_along_axis(m, q, 1))
+        q = np.take_along_axis(q, q, 1)
+    length = np.bincount((np.arange(K)[:, None] * W + m).ravel(), minlength=K * W).reshape(K, W)
+    return ((length % 2 == 1) & (m == x[None, :]) & inside).sum(axis=1)
+
+
+def _block_swap_rows(cur: np.ndarray, i, j, k) -> np.ndarray:
+    """pi[:i] + pi[j:k] + pi[i:j] + pi[k:] for every row at once."""
+    x = np.arange(cur.shape[1])[None, :]
+    i, j, k = (np.asarray(v)[:, None] for v in (i, j, k))
+    head = k - j
+    src = np.where((x < i) | (x >= k), x, np.where(x < i + head, j + (x - i), x - head))
+    return np.take_along_axis(cur, src, 1)
+
+
+def sort_rows(slot: int, rows: np.ndarray, ns, max_ratio: float, max_depth: int, e0=None, norms=None,
+              flags: int = 0) -> LadderResult:
+    """sort_batch on one-line rows (count, 64) uint8 with pi the identity;
+    e0 (odd cycles) and norms (3-norms) default to those of the rows."""
+    count = len(ns)
+    ns = np.asarray(ns, np.int32)
+    rows = np.ascontiguousarray(rows, np.uint8)
+    min_rate = float(np.float32(2.0) / np.float32(max_ratio))
+    e0 = odd_cycle_counts(rows, ns) if e0 is None else np.asarray(e0, np.int64)
+    norms = (ns - e0) // 2 if norms is None else np.asarray(norms, np.int64)
+    # SortOrExtend.java:91-105 in float32: ub = floor(3-norm * (2 / minRate))
+    ub = np.floor(norms.astype(np.float32) * (np.float32(2.0) / np.float32(min_rate)))
+    rungs = np.stack([np.full(count, 4.0), np.minimum(ub, 8), np.minimum(ub, max_depth)], 1).astype(np.int32)
+    nrungs = 1 + (ub > 4) + (ub > 8)
+    ident = np.ascontiguousarray(np.broadcast_to(np.arange(64, dtype=np.uint8), (count, 64)))
     found = np.zeros(count, bool)
     tried = np.zeros(count, np.int32)
     moves: List[Optional[List[Move]]] = [None] * count
     launches = 0
     for rung in range(3):
-        todo = [c for c in range(count) if not found[c] and rung < len(ladders[c])]
-        if not todo:
+        todo = np.flatnonzero(~found & (rung < nrungs))
+        if todo.size == 0:
             break
-        res = search_batch(slot, [list(range(len(spis[c]))) for c in todo], [spis[c] for c in todo],
-                           [e0s[c] for c in todo], min_rate, [ladders[c][rung] for c in todo], 0, flags=flags)
+        res = search_batch_rows(slot, ident[todo], rows[todo], ns[todo], e0[todo], min_rate, rungs[todo, rung], 0,
+                                flags=flags)
         launches += 1
-        for q, c in enumerate(todo):
-            tried[c] = ladders[c][rung]
-            if res.found[q]:
-                found[c] = True
-                pi = list(range(len(spis[c])))
-                moves[c] = [tuple(int(x) for x in m) for m in positions_to_symbols(pi, res.moves[q])]
+        tried[todo] = rungs[todo, rung]
+        hit = np.flatnonzero(res.found)
+        found[todo[hit]] = True
+        # positions -> symbols, replayed on pi for all found configurations of
+        # each length at once (GPUSortingSearch.java:65-87)
+        depth = res.depth[hit]
+        for d in np.unique(depth):
+            grp = hit[depth == d]
+            pos = np.asarray([res.moves[q] for q in grp], np.int64).reshape(len(grp), int(d), 3)
+            cur = ident[todo[grp]].astype(np.int64)
+            sym = np.empty_like(pos)
+            for t in range(int(d)):
+                sym[:, t] = np.take_along_axis(cur, pos[:, t], 1)
+                cur = _block_swap_rows(cur, pos[:, t, 0], pos[:, t, 1], pos[:, t, 2])
+            for q, c in enumerate(todo[grp]):
+                moves[c] = [tuple(m) for m in sym[q].tolist()]
     return LadderResult(found, tried, moves, launches)
 
 

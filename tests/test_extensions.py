@@ -253,13 +253,14 @@ def test_exploration_searches_each_canonical_form_once_and_extends_only_bad_ones
             out[i, :n] = perm(C.of_signature(sig), n)
         return None, None, None, out
 
-    def sort_batch(slot, configs, ratio, depth, flags=0):
+    def sort_rows(slot, rows, ns, ratio, depth, e0=None, norms=None, flags=0):
+        configs = [C.cycles_of([int(x) for x in rows[i, :ns[i]]]) for i in range(len(ns))]
         searched.extend(tuple(perm(c, E.size(c))) for c in configs)
         found = np.array([any(float(v) % 1 for v in E.signature(c)) for c in configs])
         return PD.LadderResult(found, np.full(len(configs), 4), [[(0, 1, 2)] if f else None for f in found], 1)
 
     monkeypatch.setattr(K, "canonical_rows", canon)
-    monkeypatch.setattr(PD, "sort_batch", sort_batch)
+    monkeypatch.setattr(PD, "sort_rows", sort_rows)
     res = E.explore([E.INTERLEAVING_PAIR, E.INTERLEAVING_PAIR], E.dfs_extension_rows, 2)
     assert len(searched) == len(set(searched)) == len(res.verdicts)
     assert res.levels[0].configurations == 1 and res.levels[0].extended == 1

@@ -125,6 +125,41 @@ def test_search_for_sorting_walks_the_ladder():
     assert gpu.calls[0] == 4 and gpu.calls[-1] == 8
 
 
+def test_batched_ladder_on_the_oracle_equals_the_sequential_ladder(monkeypatch):
+    """sort_batch (rows, one launch per rung, vectorised odd-cycle counts,
+    bounds and positions -> symbols) with the oracle standing in for the
+    batched search: the same verdicts, rungs and symbol moves as
+    SortOrExtend.searchForSorting run one configuration at a time."""
+    from tdp1375_amd.gpu_sorting_search import BatchResult
+    calls = []
+
+    def fake_rows(slot, P, S, ns, e0, min_rate, depths, full, flags=0, want_moves=None):
+        calls.append(len(ns))
+        res = [oracle.search(list(P[c, :ns[c]]), list(S[c, :ns[c]]), int(e0[c]), min_rate, int(depths[c]))
+               for c in range(len(ns))]
+        return BatchResult(np.array([r.found for r in res]), np.array([r.depth for r in res], np.int32),
+                           [r.moves for r in res], {})
+
+    monkeypatch.setattr(PD, "search_batch_rows", fake_rows)
+    data = load_cases()
+    small = sorted(data["comb"], key=lambda e: sum(len(c) for c in e["spi"]))[:24]
+    # configurations without an 11/8-sequence take every rung of their ladder
+    # (index.html:44-59): depths 4 and 5 for n = 12, 4 and 6 for n = 15
+    bad = ["(0 9 2)(1 11 4)(3 8 6)(5 10 7)", "(0 10 2)(1 5 3)(4 8 6)(7 11 9)", "(0 7 5)(1 11 9)(2 6 4)(3 10 8)",
+           "(0 4 2)(1 14 12)(3 7 5)(6 10 8)(9 13 11)"]
+    small = [{"spi": [[int(x) for x in c.split()] for c in b.strip("()").split(")(")]} for b in bad] + small
+    for entries, ratio in ((data["3_2"], 1.5), (small, 1.375)):
+        configs = [[tuple(c) for c in e["spi"]] for e in entries]
+        res = PD.sort_batch(0, configs, ratio, 11)
+        for c, cfg in enumerate(configs):
+            gpu = OracleSearch()
+            moves = PD.SortOrExtend(gpu, ratio, 11, "/tmp/tdp_driver_unused").search_for_sorting(cfg)
+            assert bool(res.found[c]) == (moves is not None), c
+            assert res.depth_tried[c] == gpu.calls[-1], c
+            assert res.moves[c] == (moves if moves is not None else None), c
+    assert calls and max(calls) <= 28
+
+
 @pytest.mark.gpu
 def test_batched_ladder_finds_every_case_file_configuration(gpu_lib):
     """Every configuration of cases-comb.txt (ratio 1.375) and cases-3,2.txt
