@@ -175,7 +175,7 @@ def search_batch_rows(slot: int, P: np.ndarray, S: np.ndarray, ns, initial_even_
     arr = np.frombuffer(out, dtype=_CASE_DTYPE, count=count)
     found = arr["found"] != 0
     depth = arr["depth"].astype(np.int32)
-    mv = arr["moves"].reshape(count, -1, 3)
+    mv = arr["moves"].reshape(count, _lib.MAX_DEPTH, 3)
     sel = np.ones(count, bool) if want_moves is None else np.asarray(want_moves, bool)
     moves = [[tuple(m) for m in mv[c, :max(int(depth[c]), 0)].tolist()] if sel[c] else []
              for c in range(count)]

@@ -220,6 +220,30 @@ def test_invalid_arguments(slots):
     assert T.search_for_sorting_seq([0, 1], [1, 0], 0, 1.0, 4, False, slots[0]).positions == []
 
 
+def test_largest_size_and_empty_batch(slots):
+    """n = 63 (MAX_N): 21 oriented 3-cycles give a 2-move at depth 1; random
+    n = 63 configurations at depth 2 (39,711 children per node) return
+    witnesses that replay; an empty batch returns empty results."""
+    empty = T.search_batch(slots[0], [], [], [], MIN_RATE_1375, 4, 0)
+    assert len(empty.found) == 0 and empty.moves == []
+    n = 63
+    pi = list(range(n))
+    spi = [3 * (x // 3) + (x % 3 + 1) % 3 for x in range(n)]          # (0 1 2)(3 4 5)...
+    e0 = W.odd_cycle_count(spi)
+    r = T.search_for_sorting_seq(pi, spi, e0, MIN_RATE_1375, 4, False, slots[0])
+    assert len(r.positions) == 1
+    assert_witness(pi, spi, e0, MIN_RATE_1375, 4, r.positions)
+    rng = W.SplitMix64(63)
+    cases = []
+    for _ in range(8):
+        p = [0] + rng.shuffle(list(range(1, n)))
+        cases.append(W.Case(p, W.spi_from_pi(p)))
+    got = gpu_batch(slots[0], cases, MIN_RATE_1375, 2, 0)
+    for c, case in enumerate(cases):
+        if got.found[c]:
+            assert_witness(case.pi, case.spi, case.even_cycles, MIN_RATE_1375, 2, got.moves[c])
+
+
 def test_facade_returns_symbol_triples(slots):
     """GPUSortingSearch.search returns symbols replayed from positions (GPUSortingSearch.java:65-75)."""
     g = T.GPUSortingSearch.__new__(T.GPUSortingSearch)
