@@ -149,11 +149,11 @@ _CASE_DTYPE = np.dtype(_lib.CaseResult)
 
 
 def search_batch_rows(slot: int, P: np.ndarray, S: np.ndarray, ns, initial_even_cycles, min_rates, max_moves,
-                      full_sorting=None, flags: int = 0, want_moves=None) -> BatchResult:
+                      full_sorting=None, flags: int = 0) -> BatchResult:
     """search_batch on pre-packed rows: P and S (count, 64) uint8 (pi and spi
-    of configuration c in their first ns[c] entries).  Moves are returned as
-    position triples for the configurations want_moves selects (a boolean
-    mask; default all), an empty list for the others."""
+    of configuration c in their first ns[c] entries); moves are position
+    triples, read with the found flags, depths and counters through a numpy
+    view of the result array."""
     count = len(ns)
     P = np.ascontiguousarray(P, np.uint8)
     S = np.ascontiguousarray(S, np.uint8)
@@ -176,9 +176,7 @@ def search_batch_rows(slot: int, P: np.ndarray, S: np.ndarray, ns, initial_even_
     found = arr["found"] != 0
     depth = arr["depth"].astype(np.int32)
     mv = arr["moves"].reshape(count, _lib.MAX_DEPTH, 3)
-    sel = np.ones(count, bool) if want_moves is None else np.asarray(want_moves, bool)
-    moves = [[tuple(m) for m in mv[c, :max(int(depth[c]), 0)].tolist()] if sel[c] else []
-             for c in range(count)]
+    moves = [[tuple(m) for m in mv[c, :max(int(depth[c]), 0)].tolist()] for c in range(count)]
     counters = {name: arr["counters"][name].astype(np.uint64) for name, _ in _lib.Counters._fields_}
     return BatchResult(found, depth, moves, counters)
 

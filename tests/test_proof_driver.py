@@ -133,7 +133,7 @@ def test_batched_ladder_on_the_oracle_equals_the_sequential_ladder(monkeypatch):
     from tdp1375_amd.gpu_sorting_search import BatchResult
     calls = []
 
-    def fake_rows(slot, P, S, ns, e0, min_rate, depths, full, flags=0, want_moves=None):
+    def fake_rows(slot, P, S, ns, e0, min_rate, depths, full, flags=0):
         calls.append(len(ns))
         res = [oracle.search(list(P[c, :ns[c]]), list(S[c, :ns[c]]), int(e0[c]), min_rate, int(depths[c]))
                for c in range(len(ns))]
