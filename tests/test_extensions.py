@@ -306,3 +306,33 @@ def test_combination_tree_runs_to_completion_on_the_gpu(slots):
     assert res.levels[-1].extended == 0                  # the last level has a sorting for everything
     assert max(l.max_bad_three_norm for l in res.levels) == 7
     assert res.levels[0].extended == 6                   # every bad small component lacks a sorting
+
+
+@pytest.mark.gpu
+def test_dfs_verdicts_equal_the_oracle_ladder(slots):
+    """Every canonical form of the first four DFS levels (6,249) gets the same
+    verdict from the batched GPU ladder as from the oracle's search run
+    through the same depth ladder (SortOrExtend.java:89-105) on the CPU."""
+    import oracle
+    res = E.explore(E.DFS_ROOTS, E.dfs_extension_rows, 4, 1.375, 11, slot=slots[0])
+    keys = list(res.verdicts)
+    assert len(keys) == 3 + 20 + 535 + 5691
+    ns = np.array([len(k) for k in keys], np.int32)
+    rows = np.zeros((len(keys), 64), np.uint8)
+    for c, k in enumerate(keys):
+        rows[c, :len(k)] = k
+    e0 = PD.odd_cycle_counts(rows, ns)
+    min_rate = float(np.float32(2) / np.float32(1.375))
+    ub = np.floor(((ns - e0) // 2).astype(np.float32) * (np.float32(2) / np.float32(min_rate)))
+    ladders = [PD.depth_ladder(float(u), 11) for u in ub]
+    found = np.zeros(len(keys), bool)
+    ident = np.broadcast_to(np.arange(64, dtype=np.uint8), (len(keys), 64))
+    for rung in range(3):
+        todo = [c for c in range(len(keys)) if not found[c] and rung < len(ladders[c])]
+        if not todo:
+            break
+        out, _ = oracle.search_many(ident[todo], rows[todo], ns[todo], e0[todo], [min_rate] * len(todo),
+                                    [ladders[c][rung] for c in todo], [0] * len(todo), True, 0, 8)
+        for q, c in enumerate(todo):
+            found[c] |= out[q].found
+    assert [res.verdicts[k] for k in keys] == found.tolist()
