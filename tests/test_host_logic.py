@@ -140,3 +140,27 @@ def test_plus2_children_stay_in_one_cycle_or_two_even_cycles():
         bound = sum(len(c) * (len(c) - 1) * (len(c) - 2) // 6 for c, a in zip(cycles, ascents) if a >= 2)
         bound += sum(len(cycles[c]) * (len(cycles[c]) - 1) // 2 * (E - len(cycles[c])) for c in even)
         assert plus2 <= bound
+
+
+def test_the_package_never_imports_the_oracle():
+    """The oracle is the tests' checker only: no module of the package
+    imports it, and importing the package (all modules) leaves it unloaded."""
+    import ast
+    import glob
+    import os
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    for path in glob.glob(os.path.join(root, "tdp1375_amd", "**", "*.py"), recursive=True):
+        tree = ast.parse(open(path).read())
+        for node in ast.walk(tree):
+            names = [a.name for a in node.names] if isinstance(node, ast.Import) else \
+                [node.module or ""] if isinstance(node, ast.ImportFrom) else []
+            assert not any(n == "oracle" or n.startswith("oracle.") for n in names), path
+    mods = [os.path.splitext(os.path.basename(p))[0] for p in glob.glob(os.path.join(root, "tdp1375_amd", "*.py"))
+            if not p.endswith("__init__.py")]
+    code = ("import sys, importlib\n" + "".join(f"importlib.import_module('tdp1375_amd.{m}')\n" for m in mods) +
+            "print(any(k == 'oracle' or k.startswith('oracle.') for k in sys.modules))")
+    out = subprocess.run([sys.executable, "-c", code], cwd=root, capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    assert out.stdout.strip() == "False"
