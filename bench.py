@@ -51,14 +51,14 @@ def algorithmic_bytes(stats: dict, counters: dict) -> int:
 
 
 def measured_traffic(workload: str):
-    """HBM bytes per search launch from the committed rocprofv3 counter
-    summary of this workload (FETCH_SIZE + WRITE_SIZE, scaled by the copy
-    calibration), or None when it is absent."""
+    """HBM bytes per search_kernel dispatch from the committed rocprofv3
+    counter summary of this workload (FETCH_SIZE + WRITE_SIZE, scaled by the
+    copy calibration), or None when it is absent."""
     path = os.path.join(ROOT, "profiles", f"search_kernel_{workload}_hbm_bytes.json")
     if not os.path.exists(path):
         return None
     with open(path) as fh:
-        return json.load(fh).get("hbm_bytes_per_launch")
+        return json.load(fh).get("hbm_bytes_per_dispatch")
 
 
 def copy_calibration(mb: int) -> None:
@@ -169,8 +169,14 @@ def main():
     T.destroy_slots()
 
     ms_per_step = 1e3 * wall / args.steps
-    avg_kernel_s = 1e-3 * sum(kernel_ms) / len(kernel_ms)
-    achieved = (sum(traffic) / len(traffic)) / avg_kernel_s / 1e9
+    # a step runs the batch as consecutive launches of at most max_active
+    # configurations; the roofline is quoted per launch (dispatch), the unit
+    # of the kernel trace's average duration
+    max_active = int(tuning.get("max_active") or autotune.DEFAULT["max_active"])
+    dispatches = -(-len(cases) // max_active)
+    avg_kernel_s = 1e-3 * sum(kernel_ms) / len(kernel_ms) / dispatches
+    bytes_per_dispatch = sum(traffic) / len(traffic) / dispatches
+    achieved = bytes_per_dispatch / avg_kernel_s / 1e9
     cases_per_hour = len(cases) * args.steps / wall * 3600.0
     out = {
         "metric": METRIC,
@@ -193,8 +199,9 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": measured_traffic(wl.name) if args.ordering == "plain" else None,
-                     "kernel": "search_kernel", "avg_kernel_ms": 1e3 * avg_kernel_s,
-                     "algorithmic_bytes_per_launch": sum(traffic) / len(traffic)},
+                     "kernel": "search_kernel", "dispatches_per_step": dispatches,
+                     "avg_kernel_ms": 1e3 * avg_kernel_s,
+                     "algorithmic_bytes_per_launch": bytes_per_dispatch},
     }
     if not args.no_cpu_baseline:
         threads = max(1, min(16, os.cpu_count() or 1))
