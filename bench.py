@@ -3,7 +3,7 @@
 A step is one plain search over the whole depth-11 batch: 4,096
 configurations, n uniform in 9..36, 3..12 odd cycles of lengths 3/5/7 (30 %
 with an oriented 3-cycle), max_ratio 1.375, max_depth 11, dedup table of 2^27
-entries, run by the library as consecutive persistent launches of at most 512
+entries, run by the library as consecutive persistent launches of at most 1,024
 configurations.  The configurations are uploaded once before the timed region
 and stay resident in HBM (TDP_FLAG_RESIDENT).
 
@@ -96,6 +96,8 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=192)
     ap.add_argument("--cpu-cap", type=int, default=40000)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB")
+    ap.add_argument("--graph", action="store_true",
+                    help="run each launch round as the slot's HIP graph (TDP_FLAG_GRAPH)")
     ap.add_argument("--ordering", choices=("plain", "learned"), default="plain",
                     help="learned: order children by the trained MLP in --model")
     ap.add_argument("--model", default=os.path.join(ROOT, "tdp1375_amd", "data", "ordering_mlp_d11.npz"))
@@ -136,6 +138,8 @@ def main():
         from tdp1375_amd import train_ordering
         train_ordering.load(args.model)
         order_flag = T._lib.FLAG_LEARNED
+    if args.graph:
+        order_flag |= T._lib.FLAG_GRAPH
 
     def step(resident: bool):
         return T.search_batch(0, pis, spis, e0s, wl.min_rate, wl.max_depth, 0,
@@ -193,7 +197,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": wl.name, "cases": len(cases), "n_range": [9, 36], "max_ratio": wl.max_ratio,
                    "max_depth": wl.max_depth, "dedup_table_size": 1 << args.dedup_log2,
-                   "queue_mb": args.queue_mb, "ordering": args.ordering, "found": int(verdicts.sum()),
+                   "queue_mb": args.queue_mb, "ordering": args.ordering, "graph": args.graph, "found": int(verdicts.sum()),
                    "tuning": tuning},
         "cases_per_hour": cases_per_hour,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -94,6 +94,9 @@ typedef struct tdp_launch_stats {
                                     /* depth -1 (undecided) instead of 0            */
 #define TDP_FLAG_LEARNED       8u   /* order each node's admitted children by the */
                                     /* move-ordering model (tdp_mlp_*), best first */
+#define TDP_FLAG_GRAPH         32u  /* run each launch round (reset, search,      */
+                                    /* control-line copy) as the slot's HIP graph */
+                                    /* (instantiated once, updated per round)     */
 #define TDP_FLAG_RESIDENT      4u   /* tdp_search_batch: reuse the configurations */
                                     /* already in device memory from the previous */
                                     /* batch on this slot (same count); the host  */

@@ -26,6 +26,7 @@ TDP_ERR_INTERNAL = 6
 FLAG_DETERMINISTIC = 1
 FLAG_SINGLE_WAVE = 2
 FLAG_RESIDENT = 4
+FLAG_GRAPH = 32
 FLAG_LEARNED = 8
 FLAG_SOFT_DEADLINE = 16
 

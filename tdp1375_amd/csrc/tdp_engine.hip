@@ -456,6 +456,7 @@ struct Slot {
     size_t cfg_cap = 0;
 
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipGraphExec_t graph_exec = nullptr;   // TDP_FLAG_GRAPH: one launch round, updated per round
     tdp_launch_stats stats{};
 
     // configurations of the last batch, reused by TDP_FLAG_RESIDENT
@@ -491,6 +492,8 @@ void destroy_slot(Slot &s) {
     s.h_results.reset();
     s.h_ctrl.reset();
     s.cancel_one.reset();
+    if (s.graph_exec) (void)hipGraphExecDestroy(s.graph_exec);
+    s.graph_exec = nullptr;
     if (s.ev0) (void)hipEventDestroy(s.ev0);
     if (s.ev1) (void)hipEventDestroy(s.ev1);
     if (s.stream) (void)hipStreamDestroy(s.stream);
@@ -614,6 +617,7 @@ void run_launch(Slot &s, const std::vector<CaseInput> &cases, uint32_t flags, bo
     const bool resident = (flags & TDP_FLAG_RESIDENT) != 0;
     const bool learned = (flags & TDP_FLAG_LEARNED) != 0;
     const bool soft = (flags & TDP_FLAG_SOFT_DEADLINE) != 0;
+    const bool graph = (flags & TDP_FLAG_GRAPH) != 0;
     s.undecided.assign((size_t)count, 0);
     HIP_CHECK(hipSetDevice(s.device));
     DeviceShared &dev = device_shared(s.device);
@@ -753,17 +757,61 @@ void run_launch(Slot &s, const std::vector<CaseInput> &cases, uint32_t flags, bo
         P.epoch = s.epoch ? s.epoch : 1;
         // one thread per configuration, wave (inbox line), pool slot and control line
         const size_t reset_n = std::max<size_t>({(size_t)sub, (size_t)waves, (size_t)pool_cap, (size_t)kNumControl});
-        hipLaunchKernelGGL(reset_kernel, dim3((unsigned)((reset_n + 255) / 256)), dim3(256), 0, s.stream, P);
-        HIP_CHECK(hipGetLastError());
-        HIP_CHECK(hipEventRecord(s.ev0, s.stream));
-        if (learned)
-            hipLaunchKernelGGL(search_learned_kernel, dim3(blocks), dim3(256), 0, s.stream, P);
-        else
-            hipLaunchKernelGGL(search_kernel, dim3(blocks), dim3(256), 0, s.stream, P);
-        HIP_CHECK(hipGetLastError());
-        HIP_CHECK(hipEventRecord(s.ev1, s.stream));
-        HIP_CHECK(hipMemcpyAsync(s.h_ctrl.get(), s.ctrl.get(), sizeof(ControlLine) * kNumControl,
-                                 hipMemcpyDeviceToHost, s.stream));
+        const auto launch_search = [&] {
+            if (learned)
+                hipLaunchKernelGGL(search_learned_kernel, dim3(blocks), dim3(256), 0, s.stream, P);
+            else
+                hipLaunchKernelGGL(search_kernel, dim3(blocks), dim3(256), 0, s.stream, P);
+            HIP_CHECK(hipGetLastError());
+        };
+        if (graph) {
+            // The round (reset kernel, search kernel, control-line copy) as a
+            // graph: captured each round and applied to the slot's executable
+            // graph by hipGraphExecUpdate (same topology, new parameters), so
+            // it is instantiated once per slot.  The graph holds no memset
+            // node.  The events bracket the whole graph.
+            HIP_CHECK(hipStreamBeginCapture(s.stream, hipStreamCaptureModeThreadLocal));
+            hipGraph_t g = nullptr;
+            try {
+                hipLaunchKernelGGL(reset_kernel, dim3((unsigned)((reset_n + 255) / 256)), dim3(256), 0, s.stream, P);
+                HIP_CHECK(hipGetLastError());
+                launch_search();
+                HIP_CHECK(hipMemcpyAsync(s.h_ctrl.get(), s.ctrl.get(), sizeof(ControlLine) * kNumControl,
+                                         hipMemcpyDeviceToHost, s.stream));
+            } catch (...) {
+                (void)hipStreamEndCapture(s.stream, &g);
+                if (g) (void)hipGraphDestroy(g);
+                throw;
+            }
+            HIP_CHECK(hipStreamEndCapture(s.stream, &g));
+            bool updated = false;
+            if (s.graph_exec) {
+                hipGraphNode_t bad = nullptr;
+                hipGraphExecUpdateResult ur = hipGraphExecUpdateError;
+                updated = hipGraphExecUpdate(s.graph_exec, g, &bad, &ur) == hipSuccess &&
+                          ur == hipGraphExecUpdateSuccess;
+                if (!updated) {
+                    (void)hipGetLastError();
+                    (void)hipGraphExecDestroy(s.graph_exec);
+                    s.graph_exec = nullptr;
+                }
+            }
+            hipError_t ie = hipSuccess;
+            if (!updated) ie = hipGraphInstantiate(&s.graph_exec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            HIP_CHECK(ie);
+            HIP_CHECK(hipEventRecord(s.ev0, s.stream));
+            HIP_CHECK(hipGraphLaunch(s.graph_exec, s.stream));
+            HIP_CHECK(hipEventRecord(s.ev1, s.stream));
+        } else {
+            hipLaunchKernelGGL(reset_kernel, dim3((unsigned)((reset_n + 255) / 256)), dim3(256), 0, s.stream, P);
+            HIP_CHECK(hipGetLastError());
+            HIP_CHECK(hipEventRecord(s.ev0, s.stream));
+            launch_search();
+            HIP_CHECK(hipEventRecord(s.ev1, s.stream));
+            HIP_CHECK(hipMemcpyAsync(s.h_ctrl.get(), s.ctrl.get(), sizeof(ControlLine) * kNumControl,
+                                     hipMemcpyDeviceToHost, s.stream));
+        }
 
         // wait: short spin, then poll with a small sleep; stop polling once a
         // cancel was requested (the kernel sees the device flag and drains)

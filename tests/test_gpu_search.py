@@ -259,3 +259,37 @@ def test_facade_returns_symbol_triples(slots):
         assert moves
         e, _, _ = oracle.replay_symbols(pi, spi, moves)
         assert oracle.satisfies_rate(e, len(moves), len(pi), e0, MIN_RATE_15, False)
+
+
+def test_graph_rounds_match_eager(slots):
+    """TDP_FLAG_GRAPH runs each launch round (reset kernel, search kernel,
+    control-line copy) as the slot's HIP graph, captured and applied to the
+    executable graph with hipGraphExecUpdate.  Deterministic mode: three graph
+    calls, each followed by an eager call, all equal to the oracle bit for bit
+    (verdicts, witnesses, counters).  Work-sharing mode over 4 rounds of 128
+    configurations (max_active 128): graph and eager verdicts equal, every
+    witness replays, on two graph calls with an eager call between."""
+    wl = W.workload_random_n12()
+    cases = wl.cases[:96]
+    want = oracle_all(cases, wl.min_rate, wl.max_depth)
+    det = T.FLAG_DETERMINISTIC
+    for rep in range(3):
+        got = gpu_batch(slots[0], cases, wl.min_rate, wl.max_depth, det | T._lib.FLAG_GRAPH)
+        assert_batch_parity(cases, got, want)
+        eager = gpu_batch(slots[0], cases, wl.min_rate, wl.max_depth, det)
+        assert_batch_parity(cases, eager, want)
+
+    batch = wl.cases
+    T.set_tuning(max_active=128)
+    try:
+        eager = gpu_batch(slots[1], batch, wl.min_rate, wl.max_depth, 0)
+        for rep in range(2):
+            got = gpu_batch(slots[1], batch, wl.min_rate, wl.max_depth, T._lib.FLAG_GRAPH)
+            assert np.array_equal(got.found, eager.found)
+            for c, case in enumerate(batch):
+                if got.found[c]:
+                    assert_witness(case.pi, case.spi, case.even_cycles, wl.min_rate, wl.max_depth, got.moves[c])
+            again = gpu_batch(slots[1], batch, wl.min_rate, wl.max_depth, 0)
+            assert np.array_equal(again.found, eager.found)
+    finally:
+        T.set_tuning()
