@@ -82,6 +82,14 @@ def lib():
                                      i32p, i32p, ctypes.c_int, ctypes.c_uint64, ctypes.c_int,
                                      ctypes.POINTER(_Result)]
         L.or_search_many.restype = ctypes.c_double
+        L.or_search_ex.argtypes = [u8p, u8p, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_int,
+                                   ctypes.c_int, ctypes.c_int, ctypes.c_uint64, ctypes.c_int,
+                                   ctypes.POINTER(_Result)]
+        L.or_search_many_ex.argtypes = [ctypes.c_int, u8p, u8p, i32p, i32p, ctypes.POINTER(ctypes.c_float),
+                                        i32p, i32p, ctypes.c_int, ctypes.c_uint64, ctypes.c_int, ctypes.c_int,
+                                        ctypes.POINTER(_Result)]
+        L.or_search_many_ex.restype = ctypes.c_double
+        L.or_children_even_arcs.argtypes = [u8p, u8p, ctypes.c_int, i32p]
         L.or_java_full_sort_search.argtypes = [u8p, u8p, ctypes.c_int, ctypes.c_int, ctypes.c_float,
                                                ctypes.c_int, ctypes.c_int, i32p]
         L.or_bfs_verdict.argtypes = [u8p, u8p, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_int,
@@ -127,12 +135,15 @@ def apply_move(pi: List[int], spi: List[int], i: int, j: int, k: int):
     return list(p[:n]), list(s[:n])
 
 
-def children_even(pi, spi) -> np.ndarray:
-    """Odd-cycle count of every child, lexicographic triple order."""
+def children_even(pi, spi, arcs: bool = False) -> np.ndarray:
+    """Odd-cycle count of every child, lexicographic triple order: by an O(n)
+    recount of each child, or (arcs=True) by re-linking the arcs of the three
+    touched cycles."""
     n = len(pi)
     T = max(1, triple_count(n))
     out = np.zeros(T, np.int32)
-    lib().or_children_even(_u8(pi, 64), _u8(spi, 64), n, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)))
+    fn = lib().or_children_even_arcs if arcs else lib().or_children_even
+    fn(_u8(pi, 64), _u8(spi, 64), n, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)))
     return out[: triple_count(n)]
 
 
@@ -142,19 +153,25 @@ def _unpack(r: _Result) -> SearchResult:
                         {k: int(getattr(r, k)) for k in COUNTER_NAMES})
 
 
+ARC_CHILDREN = 1   # OR_FLAG_ARC_CHILDREN
+
+
 def search(pi, spi, e0: int, min_rate: float, max_depth: int, full: bool = False,
-           use_dedup: bool = True, max_expansions: int = 0) -> SearchResult:
+           use_dedup: bool = True, max_expansions: int = 0, arcs: bool = False) -> SearchResult:
+    """Deterministic single-queue search.  arcs=True evaluates children by arc
+    re-linking instead of the O(n) recount (same results, faster)."""
     r = _Result()
-    st = lib().or_search(_u8(pi, 64), _u8(spi, 64), len(pi), e0, min_rate, max_depth, int(full),
-                         int(use_dedup), max_expansions, ctypes.byref(r))
+    st = lib().or_search_ex(_u8(pi, 64), _u8(spi, 64), len(pi), e0, min_rate, max_depth, int(full),
+                            int(use_dedup), max_expansions, ARC_CHILDREN if arcs else 0, ctypes.byref(r))
     if st != 0:
         raise ValueError(f"oracle search rejected arguments ({st})")
     return _unpack(r)
 
 
 def search_many(pis: np.ndarray, spis: np.ndarray, ns, e0s, min_rates, max_depths, fulls,
-                use_dedup: bool = True, max_expansions: int = 0, threads: int = 1):
-    """Runs independent searches on `threads` host threads; returns (results, seconds)."""
+                use_dedup: bool = True, max_expansions: int = 0, threads: int = 1, arcs: bool = False):
+    """Runs independent searches on `threads` host threads; returns (results, seconds).
+    arcs=True: children evaluated by arc re-linking (see search)."""
     count = len(ns)
     P = np.zeros((count, 64), np.uint8)
     S = np.zeros((count, 64), np.uint8)
@@ -169,11 +186,11 @@ def search_many(pis: np.ndarray, spis: np.ndarray, ns, e0s, min_rates, max_depth
     outs = (_Result * count)()
     u8p = ctypes.POINTER(ctypes.c_uint8)
     i32p = ctypes.POINTER(ctypes.c_int32)
-    secs = lib().or_search_many(count, P.ctypes.data_as(u8p), S.ctypes.data_as(u8p),
-                                ns_.ctypes.data_as(i32p), e0_.ctypes.data_as(i32p),
-                                mr_.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
-                                md_.ctypes.data_as(i32p), fu_.ctypes.data_as(i32p), int(use_dedup),
-                                max_expansions, threads, outs)
+    secs = lib().or_search_many_ex(count, P.ctypes.data_as(u8p), S.ctypes.data_as(u8p),
+                                   ns_.ctypes.data_as(i32p), e0_.ctypes.data_as(i32p),
+                                   mr_.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                                   md_.ctypes.data_as(i32p), fu_.ctypes.data_as(i32p), int(use_dedup),
+                                   max_expansions, ARC_CHILDREN if arcs else 0, threads, outs)
     return [_unpack(outs[c]) for c in range(count)], secs
 
 

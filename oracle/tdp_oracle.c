@@ -234,7 +234,8 @@ static int stack_push(node_stack *s, const or_node *x) {
     return 0;
 }
 
-static void make_child(const or_node *parent, const int32_t *tri, int t, int n, or_node *child) {
+/* even < 0: recount the child's odd cycles (O(n)); else take it as given */
+static void make_child(const or_node *parent, const int32_t *tri, int t, int n, int even, or_node *child) {
     int i = tri[3 * t], j = tri[3 * t + 1], k = tri[3 * t + 2];
     memcpy(child->spi, parent->spi, 64);
     memcpy(child->pi, parent->pi, 64);
@@ -242,7 +243,7 @@ static void make_child(const or_node *parent, const int32_t *tri, int t, int n, 
     memcpy(child->moves, parent->moves, sizeof(child->moves));
     child->moves[parent->depth] = (uint16_t)t;
     child->depth = (uint8_t)(parent->depth + 1);
-    child->even = (uint8_t)or_count_odd_cycles(child->spi, n);
+    child->even = (uint8_t)(even >= 0 ? even : or_count_odd_cycles(child->spi, n));
 }
 
 /* child odd-cycle count without building the child pi (sorting_search.cu:541-557) */
@@ -255,10 +256,100 @@ static int child_even(const or_node *parent, const int32_t *tri, int t, int n) {
     return or_count_odd_cycles(cs, n);
 }
 
+/* ------------------------------------------------------------------------ */
+/* child odd-cycle count by re-linking arcs (opt-in: OR_FLAG_ARC_CHILDREN)   */
+/*                                                                           */
+/* A second formulation of child_even, pinned against it (equal counts on    */
+/* every triple of random permutations, equal searches on whole workloads):  */
+/* spi' = spi o (a b c)^-1 changes only spi'(a) = spi(c), spi'(b) = spi(a),  */
+/* spi'(c) = spi(b).  Cutting the cycles of spi after a, b and c leaves arcs */
+/* spi(x) .. next(x), where next(x) is the first of a, b, c met on the way   */
+/* (x itself when x is alone in its cycle), of next(x)'s distance from x.  In */
+/* spi' the arc ending at y continues with the arc after tau(y), tau = (a->c, */
+/* b->a, c->b), so the new cycles through a, b, c are the cycles of the      */
+/* three-element map y -> next(tau(y)), each as long as its arcs together.   */
+/* Cycles not through a, b or c are unchanged.                               */
+
+typedef struct {
+    uint8_t cid[64];   /* smallest symbol of the symbol's cycle */
+    uint8_t pos[64];   /* steps from the cycle's smallest symbol */
+    uint8_t len[64];   /* cycle length, indexed by symbol       */
+} cycle_table;
+
+static void analyse_cycles(const uint8_t *spi, int n, cycle_table *ct) {
+    uint64_t seen = 0;
+    for (int s = 0; s < n; ++s) {
+        if (seen >> s & 1) continue;
+        int p = 0, x = s;
+        do {
+            seen |= 1ULL << x;
+            ct->cid[x] = (uint8_t)s;
+            ct->pos[x] = (uint8_t)p++;
+            x = spi[x];
+        } while (x != s);
+        x = s;
+        do { ct->len[x] = (uint8_t)p; x = spi[x]; } while (x != s);
+    }
+}
+
+static int child_even_arcs(const cycle_table *ct, int e, int a, int b, int c) {
+    static const int tau[3] = {2, 0, 1};   /* a -> c, b -> a, c -> b */
+    const int cid[3] = {ct->cid[a], ct->cid[b], ct->cid[c]};
+    const int pos[3] = {ct->pos[a], ct->pos[b], ct->pos[c]};
+    const int len[3] = {ct->len[a], ct->len[b], ct->len[c]};
+    /* next(x) and its distance: x itself at the cycle's length unless another
+     * of the three lies closer ahead on the same cycle */
+    int nxt[3], arc[3];
+    for (int x = 0; x < 3; ++x) {
+        nxt[x] = x;
+        arc[x] = len[x];
+        for (int q = 1; q < 3; ++q) {
+            const int y = (x + q) % 3;
+            if (cid[y] != cid[x]) continue;
+            int dist = pos[y] - pos[x];
+            if (dist < 0) dist += len[x];
+            if (dist < arc[x]) { nxt[x] = y; arc[x] = dist; }
+        }
+    }
+    const int before = (len[0] & 1) + (cid[1] != cid[0] ? (len[1] & 1) : 0) +
+                       (cid[2] != cid[0] && cid[2] != cid[1] ? (len[2] & 1) : 0);
+    int after = 0, visited = 0;
+    for (int x = 0; x < 3; ++x) {
+        if (visited >> x & 1) continue;
+        int total = 0, y = x;
+        do {
+            visited |= 1 << y;
+            total += arc[tau[y]];
+            y = nxt[tau[y]];
+        } while (y != x);
+        after += total & 1;
+    }
+    return e - before + after;
+}
+
+int or_children_even_arcs(const uint8_t *pi, const uint8_t *spi, int n, int32_t *out) {
+    if (n < 0 || n > OR_MAX_N) return -1;
+    cycle_table ct;
+    analyse_cycles(spi, n, &ct);
+    const int e = or_count_odd_cycles(spi, n);
+    int t = 0;
+    for (int i = 0; i < n - 2; ++i)
+        for (int j = i + 1; j < n - 1; ++j)
+            for (int k = j + 1; k < n; ++k) out[t++] = child_even_arcs(&ct, e, pi[i], pi[j], pi[k]);
+    return t;
+}
+
 int or_search(const uint8_t *pi, const uint8_t *spi, int n, int e0, float min_rate,
               int max_depth, int full, int use_dedup, uint64_t max_expansions,
               or_result *out) {
+    return or_search_ex(pi, spi, n, e0, min_rate, max_depth, full, use_dedup, max_expansions, 0, out);
+}
+
+int or_search_ex(const uint8_t *pi, const uint8_t *spi, int n, int e0, float min_rate,
+                 int max_depth, int full, int use_dedup, uint64_t max_expansions, int flags,
+                 or_result *out) {
     memset(out, 0, sizeof(*out));
+    const int arcs = (flags & OR_FLAG_ARC_CHILDREN) != 0;
     if (n > OR_MAX_N || n < 0) { out->status = -1; return -1; }
     if (max_depth <= 0 || max_depth > OR_MAX_DEPTH) { out->status = -2; return -2; }
     const int T = or_triple_count(n);
@@ -275,6 +366,17 @@ int or_search(const uint8_t *pi, const uint8_t *spi, int n, int e0, float min_ra
     if (use_dedup) dedup_init(&table, n);
     node_stack st = {0};
 
+    /* OR_FLAG_ARC_CHILDREN: the predicates' values per (odd-cycle count,
+     * depth), from the same float32 functions */
+    uint8_t acc_t[OR_MAX_DEPTH + 2][OR_MAX_N + 1], rch_t[OR_MAX_DEPTH + 2][OR_MAX_N + 1];
+    if (arcs) {
+        for (int d = 0; d <= max_depth + 1 && d <= OR_MAX_DEPTH + 1; ++d)
+            for (int e = 0; e <= n; ++e) {
+                acc_t[d][e] = (uint8_t)or_satisfies_rate(e, d, n, e0, min_rate, full);
+                rch_t[d][e] = (uint8_t)or_can_still_reach(e, d, n, e0, min_rate, max_depth, full);
+            }
+    }
+
     or_node start;
     memset(&start, 0, sizeof(start));
     memcpy(start.spi, spi, (size_t)n);
@@ -286,7 +388,7 @@ int or_search(const uint8_t *pi, const uint8_t *spi, int n, int e0, float min_ra
     for (int r = 0; r < T; ++r) {
         int tt = T - 1 - r;
         or_node child;
-        make_child(&start, tri, tt, n, &child);
+        make_child(&start, tri, tt, n, -1, &child);
         if (use_dedup) dedup_admit(&table, child.spi, 1);
         if (or_satisfies_rate(child.even, 1, n, e0, min_rate, full) ||
             (1 < max_depth && or_can_still_reach(child.even, 1, n, e0, min_rate, max_depth, full)))
@@ -313,9 +415,18 @@ int or_search(const uint8_t *pi, const uint8_t *spi, int n, int e0, float min_ra
         /* goal on child generation; the reported child is the first goal in
          * descending triple order (sorting_search.cu:533-535,580-592) */
         int goal_t = -1;
+        if (arcs) {
+            cycle_table ct;
+            analyse_cycles(node.spi, n, &ct);
+            for (int tt = T - 1; tt >= 0; --tt)
+                ce[tt] = child_even_arcs(&ct, node.even, node.pi[tri[3 * tt]], node.pi[tri[3 * tt + 1]],
+                                         node.pi[tri[3 * tt + 2]]);
+        } else {
+            for (int tt = T - 1; tt >= 0; --tt) ce[tt] = child_even(&node, tri, tt, n);
+        }
         for (int tt = T - 1; tt >= 0; --tt) {
-            ce[tt] = child_even(&node, tri, tt, n);
-            if (goal_t < 0 && or_satisfies_rate(ce[tt], cd, n, e0, min_rate, full)) goal_t = tt;
+            const int acc = arcs ? acc_t[cd][ce[tt]] : or_satisfies_rate(ce[tt], cd, n, e0, min_rate, full);
+            if (acc) { goal_t = tt; break; }
         }
         if (goal_t >= 0) {
             out->found = 1;
@@ -330,9 +441,10 @@ int or_search(const uint8_t *pi, const uint8_t *spi, int n, int e0, float min_ra
         if (cd >= max_depth) continue;
         /* pushes in descending triple order, so LIFO pops ascend */
         for (int tt = T - 1; tt >= 0; --tt) {
-            if (!or_can_still_reach(ce[tt], cd, n, e0, min_rate, max_depth, full)) continue;
+            const int rch = arcs ? rch_t[cd][ce[tt]] : or_can_still_reach(ce[tt], cd, n, e0, min_rate, max_depth, full);
+            if (!rch) continue;
             or_node child;
-            make_child(&node, tri, tt, n, &child);
+            make_child(&node, tri, tt, n, arcs ? ce[tt] : -1, &child);
             if (use_dedup) {
                 int r = dedup_admit(&table, child.spi, cd);
                 if (r == 0) { out->dominated++; continue; }
@@ -376,7 +488,7 @@ typedef struct {
     const uint8_t *pis, *spis;
     const int32_t *ns, *e0s, *max_depths, *fulls;
     const float *min_rates;
-    int use_dedup;
+    int use_dedup, flags;
     uint64_t max_expansions;
     or_result *outs;
     int next;
@@ -390,9 +502,9 @@ static void *batch_worker(void *arg) {
         int c = job->next++;
         pthread_mutex_unlock(&job->mu);
         if (c >= job->count) break;
-        or_search(job->pis + 64 * (size_t)c, job->spis + 64 * (size_t)c, job->ns[c], job->e0s[c],
-                  job->min_rates[c], job->max_depths[c], job->fulls[c], job->use_dedup,
-                  job->max_expansions, &job->outs[c]);
+        or_search_ex(job->pis + 64 * (size_t)c, job->spis + 64 * (size_t)c, job->ns[c], job->e0s[c],
+                     job->min_rates[c], job->max_depths[c], job->fulls[c], job->use_dedup,
+                     job->max_expansions, job->flags, &job->outs[c]);
     }
     return NULL;
 }
@@ -401,11 +513,19 @@ double or_search_many(int count, const uint8_t *pis, const uint8_t *spis, const 
                       const int32_t *e0s, const float *min_rates, const int32_t *max_depths,
                       const int32_t *fulls, int use_dedup, uint64_t max_expansions,
                       int threads, or_result *outs) {
+    return or_search_many_ex(count, pis, spis, ns, e0s, min_rates, max_depths, fulls, use_dedup,
+                             max_expansions, 0, threads, outs);
+}
+
+double or_search_many_ex(int count, const uint8_t *pis, const uint8_t *spis, const int32_t *ns,
+                         const int32_t *e0s, const float *min_rates, const int32_t *max_depths,
+                         const int32_t *fulls, int use_dedup, uint64_t max_expansions, int flags,
+                         int threads, or_result *outs) {
     batch_job job;
     memset(&job, 0, sizeof(job));
     job.count = count; job.pis = pis; job.spis = spis; job.ns = ns; job.e0s = e0s;
     job.max_depths = max_depths; job.fulls = fulls; job.min_rates = min_rates;
-    job.use_dedup = use_dedup; job.max_expansions = max_expansions; job.outs = outs;
+    job.use_dedup = use_dedup; job.flags = flags; job.max_expansions = max_expansions; job.outs = outs;
     pthread_mutex_init(&job.mu, NULL);
     if (threads < 1) threads = 1;
     pthread_t *th = (pthread_t *)malloc(sizeof(pthread_t) * (size_t)threads);

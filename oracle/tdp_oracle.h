@@ -65,12 +65,30 @@ int or_search(const uint8_t *pi, const uint8_t *spi, int n, int e0, float min_ra
               int max_depth, int full, int use_dedup, uint64_t max_expansions,
               or_result *out);
 
+/* or_search with flags.  OR_FLAG_ARC_CHILDREN evaluates each child's
+ * odd-cycle count by re-linking the arcs of the three touched cycles
+ * (child_even_arcs) instead of recounting the child's cycles: same results,
+ * same counters (checked against the recount on whole workloads), several
+ * times faster.  It exists for the verdict fixtures of large workloads. */
+#define OR_FLAG_ARC_CHILDREN 1
+int or_search_ex(const uint8_t *pi, const uint8_t *spi, int n, int e0, float min_rate,
+                 int max_depth, int full, int use_dedup, uint64_t max_expansions, int flags,
+                 or_result *out);
+
 /* Many independent searches over `threads` host threads (CPU baseline).
  * pis/spis are count x 64 bytes. Returns wall seconds. */
 double or_search_many(int count, const uint8_t *pis, const uint8_t *spis, const int32_t *ns,
                       const int32_t *e0s, const float *min_rates, const int32_t *max_depths,
                       const int32_t *fulls, int use_dedup, uint64_t max_expansions,
                       int threads, or_result *outs);
+
+double or_search_many_ex(int count, const uint8_t *pis, const uint8_t *spis, const int32_t *ns,
+                         const int32_t *e0s, const float *min_rates, const int32_t *max_depths,
+                         const int32_t *fulls, int use_dedup, uint64_t max_expansions, int flags,
+                         int threads, or_result *outs);
+
+/* odd-cycle count of each child by the arc re-linking of OR_FLAG_ARC_CHILDREN */
+int or_children_even_arcs(const uint8_t *pi, const uint8_t *spi, int n, int32_t *out);
 
 /* SortingSearch.java (fullSorting CPU search, no dedup).  Writes symbol
  * triples (a,b,c); returns the depth found (0 = none).  float_prune=0 is the

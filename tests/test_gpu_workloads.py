@@ -1,0 +1,108 @@
+"""GPU results over whole workloads against the oracle's committed results
+(tests/golden/make_search_verdicts.py writes them on the CPU):
+
+* the 2,048 n = 24 configurations of both four-3-cycle variants (max_ratio
+  1.375, D = 9, dedup 2^24, 3 GiB of queue memory, 2 slots): the
+  deterministic mode reproduces the oracle's verdict, sequence and every
+  counter of every configuration; the work-sharing mode its verdicts;
+* the 4,096-configuration depth-11 batch exactly as bench.py runs it (tuned,
+  work-sharing, dedup 2^27, 32 GiB of queue memory): every verdict the oracle
+  decided within its expansion budget is equal.
+
+Every returned sequence is replayed to an accepted state.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import tdp1375_amd as T
+from tdp1375_amd import autotune
+from tdp1375_amd import workloads as W
+from helpers import GOLDEN, assert_witness
+
+pytestmark = pytest.mark.gpu
+
+
+def _fixture(name):
+    return np.load(os.path.join(GOLDEN, f"search_{name}.npz"), allow_pickle=False)
+
+
+@pytest.fixture(scope="module")
+def n24_slots(gpu_lib):
+    T.init_slots(2, 1, 3072 << 20, 1 << 24)
+    yield [0, 1]
+    T.destroy_slots()
+
+
+@pytest.mark.parametrize("oriented", [True, False])
+def test_four_3cycles_deterministic_all_2048(n24_slots, oriented):
+    wl = W.workload_four_3cycles_n24(oriented)
+    fx = _fixture(wl.name)
+    names = [str(x) for x in fx["counter_names"]]
+    got = T.search_batch(n24_slots[1], [c.pi for c in wl.cases], [c.spi for c in wl.cases],
+                         [c.even_cycles for c in wl.cases], wl.min_rate, wl.max_depth, 0,
+                         flags=T.FLAG_DETERMINISTIC)
+    assert T.last_launch_stats(n24_slots[1])["evictions"] == 0
+    assert np.array_equal(got.found.astype(np.uint8), fx["found"])
+    for c in range(len(wl.cases)):
+        d = int(fx["depth"][c])
+        assert int(got.depth[c]) == d, f"case {c}: depth"
+        assert [tuple(m) for m in got.moves[c]] == [tuple(int(x) for x in fx["moves"][c, m]) for m in range(d)], \
+            f"case {c}: sequence"
+    for k, name in enumerate(names):
+        assert np.array_equal(got.counters[name].astype(np.uint64), fx["counters"][:, k]), name
+    assert int(got.counters["uninserted"].sum()) == 0
+
+
+@pytest.mark.parametrize("oriented", [True, False])
+def test_four_3cycles_parallel_verdicts_all_2048(n24_slots, oriented):
+    wl = W.workload_four_3cycles_n24(oriented)
+    fx = _fixture(wl.name)
+    got = T.search_batch(n24_slots[0], [c.pi for c in wl.cases], [c.spi for c in wl.cases],
+                         [c.even_cycles for c in wl.cases], wl.min_rate, wl.max_depth, 0, flags=0)
+    assert np.array_equal(got.found.astype(np.uint8), fx["found"])
+    for c, case in enumerate(wl.cases):
+        if got.found[c]:
+            assert_witness(case.pi, case.spi, case.even_cycles, wl.min_rate, wl.max_depth, got.moves[c])
+
+
+def _depth11_verdicts():
+    path = os.path.join(GOLDEN, "verdicts_odd-cycle-mix-n9-36-d11.json")
+    with open(path) as fh:
+        return json.load(fh)
+
+
+def run_depth11_bench_path(flags=0):
+    """The bench's step: the whole batch, tuned, work-sharing mode."""
+    wl = W.workload_odd_cycle_mix_d11(4096)
+    T.init_slots(1, 1, 32768 << 20, 1 << 27)
+    try:
+        autotune.apply(autotune.DEFAULT_CACHE, wl.name)
+        got = T.search_batch(0, [c.pi for c in wl.cases], [c.spi for c in wl.cases],
+                             [c.even_cycles for c in wl.cases], wl.min_rate, wl.max_depth, 0, flags=flags)
+    finally:
+        autotune.reset()
+        T.destroy_slots()
+    return wl, got
+
+
+def check_depth11(wl, got):
+    fx = _depth11_verdicts()
+    verdicts = fx["verdicts"]
+    assert len(verdicts) == len(wl.cases)
+    decided = [c for c, v in enumerate(verdicts) if v != "?"]
+    want = np.array([verdicts[c] == "1" for c in decided])
+    assert np.array_equal(got.found[decided], want), \
+        [c for c, w in zip(decided, want) if bool(got.found[c]) != w][:20]
+    for c, case in enumerate(wl.cases):
+        if got.found[c]:
+            assert_witness(case.pi, case.spi, case.even_cycles, wl.min_rate, wl.max_depth, got.moves[c])
+    return len(decided)
+
+
+def test_depth11_batch_bench_path_verdicts(gpu_lib):
+    wl, got = run_depth11_bench_path()
+    decided = check_depth11(wl, got)
+    assert decided >= 3900

@@ -276,3 +276,55 @@ def test_filter_oriented_5_cycle_counts_by_symmetry():
     total_skip, _ = oracle.filter_pi([0, 3, 1, 4, 2], [0, 1, 2], 5, True, 100)
     total, _ = oracle.filter_pi([0, 3, 1, 4, 2], [0, 1, 2], 5, False, 100)
     assert 0 < total <= total_skip <= 24
+
+
+def test_arc_relinking_equals_recount_on_every_triple():
+    """The fixtures' child evaluation (arc re-linking) gives the O(n)
+    recount's odd-cycle count for every triple: random permutations, sparse
+    cycle structures (mostly fixed points), single long cycles, n = 3..63."""
+    rng = np.random.default_rng(13)
+    for it in range(150):
+        n = int(rng.integers(3, 64))
+        pi = [int(x) for x in rng.permutation(n)]
+        if it % 3 == 0:
+            sym = [int(x) for x in rng.permutation(n)]
+            cyc, p = [], 0
+            while p < n // 2:
+                L = int(rng.integers(1, 8))
+                cyc.append(sym[p:p + L])
+                p += L
+            spi = W.one_line(cyc, n)
+        elif it % 3 == 1:
+            spi = W.one_line([[int(x) for x in rng.permutation(n)]], n)
+        else:
+            spi = [int(x) for x in rng.permutation(n)]
+        assert np.array_equal(oracle.children_even(pi, spi, arcs=True), oracle.children_even(pi, spi)), (it, n)
+
+
+def _same_result(a, b):
+    return (a.found, a.depth, a.moves, a.aborted, a.counters) == (b.found, b.depth, b.moves, b.aborted, b.counters)
+
+
+def test_arc_search_equals_literal_search():
+    """Whole searches with arc re-linking equal the literal ones in every
+    field (verdict, sequence, counters, budget stop): the 512 n = 12 cases
+    (D = 6), 256 of each n = 24 variant (D = 9), the bad small components
+    (D = 11) and 24 depth-11 configurations under an expansion budget."""
+    groups = []
+    wl = W.workload_random_n12()
+    groups.append((wl.cases, wl.min_rate, wl.max_depth, 0))
+    for oriented in (True, False):
+        wl = W.workload_four_3cycles_n24(oriented)
+        groups.append((wl.cases[:256], wl.min_rate, wl.max_depth, 0))
+    groups.append((W.bad_small_components(), MIN_RATE_1375, 11, 0))
+    wl = W.workload_odd_cycle_mix_d11(4096)
+    groups.append((wl.cases[::171][:24], wl.min_rate, 11, 3000))
+    for cases, rate, D, cap in groups:
+        args = ([c.pi for c in cases], [c.spi for c in cases], [c.n for c in cases], [c.even_cycles for c in cases],
+                [rate] * len(cases), [D] * len(cases), [0] * len(cases), True, cap, 8)
+        lit, _ = oracle.search_many(*args)
+        arc, _ = oracle.search_many(*args, arcs=True)
+        for c in range(len(cases)):
+            assert _same_result(lit[c], arc[c]), (D, c)
+        if cap:
+            assert any(r.aborted for r in lit) and not all(r.aborted for r in lit)
