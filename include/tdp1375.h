@@ -229,6 +229,18 @@ TDP_API int tdp_canonical_batch(int count, const uint8_t *spis, const uint8_t *p
                                 float *out_signatures, int32_t *out_hashes, int32_t *out_index,
                                 uint8_t *out_canonical_spi);
 
+/* Allocation retry timeout (replaces gGpuMemoryRetryTimeoutMs,
+ * cuda_memory_pressure.cuh:10, read by allocateDeviceBuffer,
+ * cuda_memory.cuh:72): a device allocation that fails for lack of memory is
+ * retried with backoff (0.1 s doubling to 5 s) until this many milliseconds
+ * have passed, then fails with TDP_ERR_OUT_OF_MEMORY and "HIP out of memory
+ * while allocating N bytes for <label>" (cuda_memory.cuh:82-85).  Default
+ * 60000; a negative value acts as 0 (no retry). */
+TDP_API int tdp_set_alloc_timeout(int64_t milliseconds);
+/* Allocation retries so far in this process (gGpuMemoryRetries,
+ * cuda_memory_pressure.cuh:9). */
+TDP_API uint64_t tdp_alloc_retries(void);
+
 /* Number of visible GPUs (0 when none). */
 TDP_API int tdp_device_count(void);
 

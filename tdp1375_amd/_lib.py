@@ -36,6 +36,7 @@ EXPORTED_SYMBOLS = (
     "tdp_filter_last_kernel_ms", "tdp_eval_children", "tdp_host_eval_children",
     "tdp_host_predicate_tables", "tdp_set_tuning", "tdp_set_launch_deadline", "tdp_set_expansion_batch",
     "tdp_canonical_batch",
+    "tdp_set_alloc_timeout", "tdp_alloc_retries",
     "tdp_device_count",
     "tdp_mlp_set_weights",
     "tdp_mlp_get_weights", "tdp_mlp_init", "tdp_mlp_forward", "tdp_mlp_gradients", "tdp_mlp_train",
@@ -121,6 +122,9 @@ def load() -> ctypes.CDLL:
                                  ctypes.c_int64, i64p, i64p]
         L.tdp_filter_last_kernel_ms.restype = ctypes.c_double
         L.tdp_filter_last_kernel_ms.argtypes = []
+        L.tdp_set_alloc_timeout.argtypes = [ctypes.c_int64]
+        L.tdp_alloc_retries.argtypes = []
+        L.tdp_alloc_retries.restype = ctypes.c_uint64
         L.tdp_eval_children.argtypes = [u8p, u8p, ctypes.c_int, i32p]
         L.tdp_host_eval_children.argtypes = [u8p, u8p, ctypes.c_int, i32p]
         L.tdp_eval_features.argtypes = [u8p, u8p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,

@@ -98,14 +98,19 @@ int guarded(F &&f) {
 
 // ---------------------------------------------------------------- memory
 // cuda_memory.cuh:60-99: retry hipMalloc on out-of-memory with exponential
-// backoff (100 ms doubling to 5 s) until 60 s have passed.
+// backoff (100 ms doubling to 5 s) until the retry timeout has passed
+// (gGpuMemoryRetryTimeoutMs, cuda_memory_pressure.cuh:10: 60 s by default,
+// negative values act as 0; tdp_set_alloc_timeout).  Retries are counted
+// (gGpuMemoryRetries).
 std::atomic<int64_t> g_alloc_timeout_ms{60000};
+std::atomic<uint64_t> g_alloc_retries{0};
 
 void *device_alloc(size_t bytes, const char *label) {
     if (bytes == 0) return nullptr;
     void *p = nullptr;
     const auto start = std::chrono::steady_clock::now();
     int backoff = 100;
+    const int64_t timeout_ms = std::max<int64_t>(g_alloc_timeout_ms.load(std::memory_order_relaxed), 0);
     while (true) {
         const hipError_t st = hipMalloc(&p, bytes);
         if (st == hipSuccess) return p;
@@ -113,10 +118,11 @@ void *device_alloc(size_t bytes, const char *label) {
         (void)hipGetLastError();
         const auto el = std::chrono::duration_cast<std::chrono::milliseconds>(
                             std::chrono::steady_clock::now() - start).count();
-        if (el >= g_alloc_timeout_ms.load()) {
+        if (el >= timeout_ms) {
             throw GpuMemoryExhausted(std::string("HIP out of memory while allocating ") +
                                      std::to_string(bytes) + " bytes for " + label);
         }
+        g_alloc_retries.fetch_add(1, std::memory_order_relaxed);
         std::this_thread::sleep_for(std::chrono::milliseconds(backoff));
         backoff = std::min(backoff * 2, 5000);
     }
@@ -457,6 +463,7 @@ struct Slot {
 
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     hipGraphExec_t graph_exec = nullptr;   // TDP_FLAG_GRAPH: one launch round, updated per round
+    int graph_kernel = -1;                 // search kernel in graph_exec: 0 plain, 1 learned
     tdp_launch_stats stats{};
 
     // configurations of the last batch, reused by TDP_FLAG_RESIDENT
@@ -522,8 +529,13 @@ void init_slots(int total, int devices, int64_t budget, int64_t dedup) {
     std::lock_guard<std::mutex> lk(g_slots_mutex);
     int cur = -1;
     (void)hipGetDevice(&cur);
+    // a slot that fails half-way (e.g. out of memory) is torn down before
+    // the error propagates, and so are the slots already built: a failed
+    // initSlots leaves no slots, as destroySlots would
+    std::unique_ptr<Slot> s;
+    try {
     for (int i = 0; i < total; ++i) {
-        auto s = std::make_unique<Slot>();
+        s = std::make_unique<Slot>();
         s->device = i % devices;
         HIP_CHECK(hipSetDevice(s->device));
         HIP_CHECK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
@@ -554,6 +566,14 @@ void init_slots(int total, int devices, int64_t budget, int64_t dedup) {
         }
         s->epoch = 0;
         g_slots.push_back(std::move(s));
+    }
+    } catch (...) {
+        if (s) destroy_slot(*s);
+        for (auto &o : g_slots)
+            if (o) destroy_slot(*o);
+        g_slots.clear();
+        if (cur >= 0) (void)hipSetDevice(cur);
+        throw;
     }
     if (cur >= 0) (void)hipSetDevice(cur);
 }
@@ -768,8 +788,10 @@ void run_launch(Slot &s, const std::vector<CaseInput> &cases, uint32_t flags, bo
             // The round (reset kernel, search kernel, control-line copy) as a
             // graph: captured each round and applied to the slot's executable
             // graph by hipGraphExecUpdate (same topology, new parameters), so
-            // it is instantiated once per slot.  The graph holds no memset
-            // node.  The events bracket the whole graph.
+            // it is instantiated once per slot (again when the search kernel
+            // function changes).  The graph holds no memset node.  The events
+            // bracket the whole graph, so in this mode kernel_ms is the time
+            // of the round (reset kernel + search kernel + control copy).
             HIP_CHECK(hipStreamBeginCapture(s.stream, hipStreamCaptureModeThreadLocal));
             hipGraph_t g = nullptr;
             try {
@@ -785,6 +807,12 @@ void run_launch(Slot &s, const std::vector<CaseInput> &cases, uint32_t flags, bo
             }
             HIP_CHECK(hipStreamEndCapture(s.stream, &g));
             bool updated = false;
+            if (s.graph_exec && s.graph_kernel != (learned ? 1 : 0)) {
+                // the search node runs another kernel function: instantiate
+                // anew rather than rely on an update changing the function
+                (void)hipGraphExecDestroy(s.graph_exec);
+                s.graph_exec = nullptr;
+            }
             if (s.graph_exec) {
                 hipGraphNode_t bad = nullptr;
                 hipGraphExecUpdateResult ur = hipGraphExecUpdateError;
@@ -799,7 +827,9 @@ void run_launch(Slot &s, const std::vector<CaseInput> &cases, uint32_t flags, bo
             hipError_t ie = hipSuccess;
             if (!updated) ie = hipGraphInstantiate(&s.graph_exec, g, nullptr, nullptr, 0);
             (void)hipGraphDestroy(g);
+            if (ie != hipSuccess) s.graph_exec = nullptr;
             HIP_CHECK(ie);
+            s.graph_kernel = learned ? 1 : 0;
             HIP_CHECK(hipEventRecord(s.ev0, s.stream));
             HIP_CHECK(hipGraphLaunch(s.graph_exec, s.stream));
             HIP_CHECK(hipEventRecord(s.ev1, s.stream));
@@ -927,9 +957,8 @@ struct FilterWorkspace {
 std::mutex g_filter_mutex;
 std::vector<std::unique_ptr<FilterWorkspace>> g_filter_ws;
 
-void filter_impl(const uint8_t *cycle, int L, const uint8_t *triple, int tl, int n, int skip, uint8_t *out,
-                 int64_t out_cap, int64_t *out_total, int64_t *out_written) {
-    // pi_filter.cu:165-166
+// pi_filter.cu:165-166 and the argument checks of both filter entry points
+void validate_filter_args(const uint8_t *cycle, int L, const uint8_t *triple, int tl, int n, int64_t out_cap) {
     if (n > kMaxN) throw std::invalid_argument("n exceeds PI_FILTER_MAX_N");
     if (n < 1) throw std::invalid_argument("n must be at least 1");
     if (n > 21) throw std::invalid_argument("n too large: (n-1)! exceeds 64-bit rank enumeration");
@@ -940,7 +969,10 @@ void filter_impl(const uint8_t *cycle, int L, const uint8_t *triple, int tl, int
     for (int i = 0; i < tl; ++i)
         if (triple[i] >= n) throw std::invalid_argument("triple symbol out of range");
     if (out_cap < 0) throw std::invalid_argument("negative output capacity");
+}
 
+// Parameters common to both filter entry points (everything but the buffers).
+FilterParams filter_params(const uint8_t *cycle, int L, const uint8_t *triple, int tl, int n, int skip) {
     FilterParams F{};
     F.n = n;
     F.cycle_len = L;
@@ -948,6 +980,17 @@ void filter_impl(const uint8_t *cycle, int L, const uint8_t *triple, int tl, int
     F.skip_2move = skip ? 1 : 0;
     std::memcpy(F.cycle, cycle, (size_t)L);
     std::memcpy(F.triple, triple, (size_t)tl);
+    F.fact[0] = 1;
+    for (int i = 1; i < 24; ++i) F.fact[i] = F.fact[i - 1] * (uint64_t)(i < 21 ? i : 1);
+    F.rank32 = F.fact[n - 1] < (1ULL << 32) ? 1 : 0;
+    return F;
+}
+
+void filter_impl(const uint8_t *cycle, int L, const uint8_t *triple, int tl, int n, int skip, uint8_t *out,
+                 int64_t out_cap, int64_t *out_total, int64_t *out_written) {
+    validate_filter_args(cycle, L, triple, tl, n, out_cap);
+
+    FilterParams F = filter_params(cycle, L, triple, tl, n, skip);
     // 2-move candidates of the cycle (pi_filter.cu:86-115): triples of cycle
     // indices i<j<k whose arcs (taken through the symbol -> index map as the
     // reference does) turn `before` into `before + 2` odd cycles.
@@ -967,10 +1010,7 @@ void filter_impl(const uint8_t *cycle, int L, const uint8_t *triple, int tl, int
                 if (after - before == 2) moves.push_back((uint32_t)a | ((uint32_t)b << 8) | ((uint32_t)c << 16));
             }
     F.move_count = (int)moves.size();
-    F.fact[0] = 1;
-    for (int i = 1; i < 24; ++i) F.fact[i] = F.fact[i - 1] * (uint64_t)(i < 21 ? i : 1);
     const uint64_t total = F.fact[n - 1];
-    F.rank32 = total < (1ULL << 32) ? 1 : 0;
 
     int device = 0;
     HIP_CHECK(hipGetDevice(&device));
@@ -1108,6 +1148,13 @@ int tdp_set_tuning(int waves_per_cu, int share_interval, int share_max, int idle
         g_tuning = t;
     });
 }
+
+int tdp_set_alloc_timeout(int64_t milliseconds) {
+    g_alloc_timeout_ms.store(milliseconds, std::memory_order_relaxed);
+    return TDP_OK;
+}
+
+uint64_t tdp_alloc_retries(void) { return g_alloc_retries.load(std::memory_order_relaxed); }
 
 int tdp_device_count(void) {
     int c = 0;

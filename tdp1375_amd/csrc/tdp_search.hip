@@ -747,12 +747,18 @@ __device__ __forceinline__ int dedup_admit(const SearchParams &P, WaveLds &L, in
                     if (w < nw2) eq &= kv[w] == key_pair(L, lane, w, nw);
                 count_active(&L.traffic[3], __popcll(__builtin_amdgcn_read_exec()));
                 if (!eq) continue;
-                // The snapshot tag decides: it was an admission of this key at
-                // its depth.  No entry is replaced by one of the same identity
-                // (victim choice below), so a key rewritten by an eviction
-                // meanwhile has another identity and stands in for the
-                // snapshot's only under a 27-bit hash collision, the odds of
-                // an ABA on the tag word.
+                // The snapshot tag decides only if it still holds after the
+                // key was read: an entry evicted and rewritten meanwhile (by
+                // a key of the same 27-bit identity) changes the tag word, and
+                // the lookup starts over instead of judging this key by the
+                // depth of the one the snapshot described.  What remains is
+                // an ABA of the whole tag word (same identity, same depth,
+                // ready again) inside the window, in which case the entry
+                // holds this key at that depth or a key written after it.
+                if (ld_relaxed(tags + q) != tq) {
+                    retry = true;
+                    continue;
+                }
                 if ((int)(tq & 15ULL) <= depth) {
                     res = kDedupDominated;
                 } else if (cas64(tags + q, tq, (tq & ~15ULL) | (unsigned long long)depth)) {

@@ -182,3 +182,30 @@ def test_jni_round_trip_on_gpu(gpu_lib):
     finally:
         destroy(env.env, None)
     assert env.exception is None
+
+
+@pytest.mark.gpu
+def test_jni_out_of_memory_becomes_runtime_exception(gpu_lib):
+    """initSlots with a queue budget no device holds throws RuntimeException
+    with the allocation message (jni_sorter.cu:96-103, cuda_memory.cuh:82-85);
+    initSlots works again afterwards."""
+    init, destroy, search, cancel, filt = natives()
+    env = FakeJNIEnv()
+    _lib.check(gpu_lib.tdp_set_alloc_timeout(0))
+    try:
+        init(env.env, None, 2, 1, 1 << 40, 1 << 16)
+        assert env.exception == ("java/lang/RuntimeException",
+                                 f"HIP out of memory while allocating {1 << 40} bytes for sort.nodes")
+    finally:
+        _lib.check(gpu_lib.tdp_set_alloc_timeout(60000))
+    env.exception = None
+    init(env.env, None, 1, 1, 64 << 20, 1 << 20)
+    assert env.exception is None
+    try:
+        wl = W.workload_random_n12()
+        case = wl.cases[1]
+        r = search(env.env, None, env.byte_array(case.pi), env.byte_array(case.spi), case.even_cycles,
+                   wl.min_rate, 6, 0, 0)
+        assert env.exception is None and r is not None
+    finally:
+        destroy(env.env, None)

@@ -2,6 +2,7 @@
 the CPU reference, training on the GPU, and the search with the learned
 ordering (same verdicts as the plain search, fewer expansions to verdict)."""
 import ctypes
+import os
 
 import numpy as np
 import pytest
@@ -207,3 +208,28 @@ def test_learned_ordering_keeps_verdicts(slots):
     le = lrn.counters["expansions"][sat].astype(np.float64)
     print(f"satisfiable {int(sat.sum())}: expansions to verdict plain {pe.sum():.0f} learned {le.sum():.0f} "
           f"(ratio {pe.sum() / max(le.sum(), 1):.2f})", flush=True)
+
+
+@pytest.mark.gpu
+def test_graph_switches_between_plain_and_learned_kernels(slots):
+    """One slot alternates graph rounds of the plain and the learned search
+    kernel, with the number of rounds (and so the reset grid) changing between
+    calls: each graph call equals its eager twin (deterministic mode: verdict,
+    sequence and counters)."""
+    from tdp1375_amd import train_ordering
+    train_ordering.load(os.path.join(os.path.dirname(LN.__file__), "data", "ordering_mlp_d11.npz"))
+    wl = W.workload_random_n12()
+    det = T.FLAG_DETERMINISTIC
+    G = _lib.FLAG_GRAPH
+    plan = [(0, 96), (_lib.FLAG_LEARNED, 96), (0, 40), (_lib.FLAG_LEARNED, 200), (_lib.FLAG_LEARNED, 40), (0, 200)]
+    for extra, count in plan:
+        cases = wl.cases[:count]
+        args = ([c.pi for c in cases], [c.spi for c in cases], [c.even_cycles for c in cases], wl.min_rate,
+                wl.max_depth, 0)
+        got = T.search_batch(slots[1], *args, flags=det | extra | G)
+        eager = T.search_batch(slots[1], *args, flags=det | extra)
+        assert np.array_equal(got.found, eager.found), (extra, count)
+        assert np.array_equal(got.depth, eager.depth), (extra, count)
+        assert [list(map(tuple, m)) for m in got.moves] == [list(map(tuple, m)) for m in eager.moves]
+        for name in got.counters:
+            assert np.array_equal(got.counters[name], eager.counters[name]), (extra, count, name)
