@@ -151,6 +151,22 @@ TDP_API int tdp_filter(const uint8_t *cycle, int cycle_len, const uint8_t *tripl
                int skip_2move_check, uint8_t *out, int64_t out_rows_cap, int64_t *out_total,
                int64_t *out_written);
 
+/* tdp_filter without any allocation (no counterpart in the reference; for
+ * callers that cannot allocate device memory: code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same arguments,
+ * checks and results as tdp_filter, but the outputs are in device memory and
+ * the call only enqueues work on `stream` (a hipStream_t; NULL = the null
+ * stream) and returns: out_rows_dev receives the first min(count,
+ * out_rows_cap) passing rows (n bytes each, rank order), *out_total_dev the
+ * uncapped count once the stream reaches that point.  Working memory is a
+ * fixed 11.2 MiB device array inside the library (rank passes of 2^26, the
+ * cycle's 2-move table), one per GPU for the life of the process, so any
+ * valid input fits it; calls that overlap on one GPU (another thread or
+ * stream) race on that array and must be serialised by the caller. */
+TDP_API int tdp_filter_noalloc(const uint8_t *cycle, int cycle_len, const uint8_t *triple, int triple_len, int n,
+                               int skip_2move_check, uint8_t *out_rows_dev, int64_t out_rows_cap,
+                               uint64_t *out_total_dev, void *stream);
+
 /* Kernel time of the last tdp_filter call on this thread, milliseconds. */
 TDP_API double tdp_filter_last_kernel_ms(void);
 

@@ -33,7 +33,7 @@ FLAG_SOFT_DEADLINE = 16
 EXPORTED_SYMBOLS = (
     "tdp_last_error", "tdp_init_slots", "tdp_destroy_slots", "tdp_search", "tdp_search_ex",
     "tdp_search_batch", "tdp_last_launch_stats", "tdp_cancel", "tdp_filter", "tdp_eval_features",
-    "tdp_filter_last_kernel_ms", "tdp_eval_children", "tdp_host_eval_children",
+    "tdp_filter_last_kernel_ms", "tdp_filter_noalloc", "tdp_eval_children", "tdp_host_eval_children",
     "tdp_host_predicate_tables", "tdp_set_tuning", "tdp_set_launch_deadline", "tdp_set_expansion_batch",
     "tdp_canonical_batch",
     "tdp_set_alloc_timeout", "tdp_alloc_retries",
@@ -122,6 +122,8 @@ def load() -> ctypes.CDLL:
                                  ctypes.c_int64, i64p, i64p]
         L.tdp_filter_last_kernel_ms.restype = ctypes.c_double
         L.tdp_filter_last_kernel_ms.argtypes = []
+        L.tdp_filter_noalloc.argtypes = [u8p, ctypes.c_int, u8p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                         ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p]
         L.tdp_set_alloc_timeout.argtypes = [ctypes.c_int64]
         L.tdp_alloc_retries.argtypes = []
         L.tdp_alloc_retries.restype = ctypes.c_uint64

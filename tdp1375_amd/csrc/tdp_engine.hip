@@ -1083,6 +1083,86 @@ void filter_impl(const uint8_t *cycle, int L, const uint8_t *triple, int tl, int
     if (out_written) *out_written = (int64_t)rows;
 }
 
+// ---------------------------------------------------------------- allocation-free filter
+// tdp_filter_noalloc runs the filter's passes on the caller's stream with all
+// of its working memory in `scratch`, a file-scope device array: rank chunks
+// of 2^26 (8 MiB of masks, 1 MiB of block counts, 2 MiB of offsets) and the
+// 2-move table (at most C(64,3) entries) with its count.  Each device holds
+// its own copy of the array from library load to process end; host code
+// reaches it only through the address hipGetSymbolAddress returns for the
+// current device, resolved once per device on the first call there.
+constexpr uint64_t kNoAllocChunk = 1ULL << 26;                           // ranks per pass
+constexpr size_t kNoAllocBlocks = (size_t)(kNoAllocChunk / 256);         // mask kernel blocks
+constexpr size_t kNoAllocMaxMoves = 64 * 63 * 62 / 6;                    // C(64,3)
+constexpr size_t kScratchMaskOff = 0;
+constexpr size_t kScratchCountsOff = kScratchMaskOff + kNoAllocBlocks * 4 * 8;
+constexpr size_t kScratchOffsetsOff = kScratchCountsOff + kNoAllocBlocks * 4;
+constexpr size_t kScratchMovesOff = kScratchOffsetsOff + kNoAllocBlocks * 8;
+constexpr size_t kScratchMoveCountOff = kScratchMovesOff + ((kNoAllocMaxMoves * 4 + 255) & ~(size_t)255);
+constexpr size_t kScratchBytes = kScratchMoveCountOff + 256;
+
+__device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+std::mutex g_scratch_mutex;
+std::vector<unsigned char *> g_scratch_ptr;   // per device; nullptr until resolved
+
+unsigned char *scratch_for(int device) {
+    std::lock_guard<std::mutex> lk(g_scratch_mutex);
+    if ((size_t)device >= g_scratch_ptr.size()) g_scratch_ptr.resize(device + 1, nullptr);
+    if (!g_scratch_ptr[device]) {
+        void *ptr = nullptr;
+        HIP_CHECK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        if (!ptr) throw std::runtime_error("no device address for the filter scratch array");
+        g_scratch_ptr[device] = static_cast<unsigned char *>(ptr);
+    }
+    return g_scratch_ptr[device];
+}
+
+void filter_noalloc_impl(const uint8_t *cycle, int L, const uint8_t *triple, int tl, int n, int skip,
+                         uint8_t *out_rows, int64_t out_cap, uint64_t *out_total, hipStream_t stream) {
+    validate_filter_args(cycle, L, triple, tl, n, out_cap);
+    if (!out_total) throw std::invalid_argument("null argument");
+    if (out_cap > 0 && !out_rows) throw std::invalid_argument("null argument");
+    // every working buffer of this call fits the scratch array by
+    // construction: one pass takes at most kNoAllocChunk ranks and a cycle
+    // of at most 64 symbols has at most C(64,3) 2-move candidates
+    const size_t max_moves = L >= 3 ? (size_t)L * (L - 1) * (L - 2) / 6 : 0;
+    if (max_moves > kNoAllocMaxMoves) throw std::invalid_argument("cycle longer than the scratch array allows");
+    int device = 0;
+    HIP_CHECK(hipGetDevice(&device));
+    unsigned char *base = scratch_for(device);
+    FilterParams F = filter_params(cycle, L, triple, tl, n, skip);
+    uint32_t *moves = reinterpret_cast<uint32_t *>(base + kScratchMovesOff);
+    uint32_t *move_count = reinterpret_cast<uint32_t *>(base + kScratchMoveCountOff);
+    F.moves = moves;
+    F.move_count_dev = move_count;
+    F.mask = reinterpret_cast<unsigned long long *>(base + kScratchMaskOff);
+    F.block_counts = reinterpret_cast<uint32_t *>(base + kScratchCountsOff);
+    uint64_t *offsets = reinterpret_cast<uint64_t *>(base + kScratchOffsetsOff);
+    F.out = out_rows;
+    F.out_cap = (uint64_t)out_cap;
+    const uint64_t total = F.fact[n - 1];
+    // the set-up pass also clears the caller's total, which the scans then
+    // accumulate (no memset: the sequence may be captured into a graph)
+    hipLaunchKernelGGL(filter_moves_kernel, dim3(1), dim3(64), 0, stream, F, moves, move_count, out_total);
+    HIP_CHECK(hipGetLastError());
+    for (uint64_t off = 0; off < total; off += kNoAllocChunk) {
+        F.rank_offset = off;
+        F.rank_end = std::min(total, off + kNoAllocChunk);
+        const uint64_t nb = (F.rank_end - off + 255) / 256;
+        hipLaunchKernelGGL(filter_mask_kernel, dim3((unsigned)nb), dim3(256), 0, stream, F);
+        HIP_CHECK(hipGetLastError());
+        hipLaunchKernelGGL(filter_scan_kernel, dim3(1), dim3(1024), 0, stream, F.block_counts, nb, offsets,
+                           out_total);
+        HIP_CHECK(hipGetLastError());
+        if (out_cap > 0) {
+            hipLaunchKernelGGL(filter_write_kernel, dim3((unsigned)nb), dim3(256), 0, stream, F,
+                               (const uint64_t *)offsets);
+            HIP_CHECK(hipGetLastError());
+        }
+    }
+}
+
 void host_eval_children(const uint8_t *pi, const uint8_t *spi, int n, int32_t *out) {
     uint32_t info[64];
     bool seen[64] = {false};
@@ -1270,6 +1350,15 @@ int tdp_filter(const uint8_t *cycle, int cycle_len, const uint8_t *triple, int t
 }
 
 double tdp_filter_last_kernel_ms(void) { return g_filter_ms; }
+
+int tdp_filter_noalloc(const uint8_t *cycle, int cycle_len, const uint8_t *triple, int triple_len, int n,
+                       int skip_2move_check, uint8_t *out_rows_dev, int64_t out_rows_cap, uint64_t *out_total_dev,
+                       void *stream) {
+    return guarded([&] {
+        filter_noalloc_impl(cycle, cycle_len, triple, triple_len, n, skip_2move_check, out_rows_dev, out_rows_cap,
+                            out_total_dev, static_cast<hipStream_t>(stream));
+    });
+}
 
 int tdp_eval_children(const uint8_t *pi, const uint8_t *spi, int n, int32_t *out) {
     return guarded([&] {

@@ -76,7 +76,8 @@ __device__ __forceinline__ bool accepts(const FilterParams &F, PosFn pos) {
     if (leaps <= 1) return false;
     // 3. no 2-move among the host-listed cycle triples (pi_filter.cu:86-115)
     if (!F.skip_2move) {
-        for (int t = 0; t < F.move_count; ++t) {
+        const int move_count = F.move_count_dev ? (int)*F.move_count_dev : F.move_count;
+        for (int t = 0; t < move_count; ++t) {
             const uint32_t m = F.moves[t];
             const int ia = pos(m & 0xFF), ib = pos((m >> 8) & 0xFF), ic = pos((m >> 16) & 0xFF);
             if ((ia > ib) + (ib > ic) + (ic > ia) <= 1) return false;
@@ -190,6 +191,49 @@ __global__ __launch_bounds__(kBlock) void filter_write_kernel(FilterParams F, co
     rank_row(F, r, row);
     uint8_t *dst = F.out + idx * (uint64_t)F.n;
     for (int p = 0; p < F.n; ++p) dst[p] = row[p];
+}
+
+// The host list of tdp_filter as a kernel (pi_filter.cu:86-115): triples
+// i<j<k of cycle indices whose arcs, measured through the symbol -> index
+// map, turn the cycle's odd count `before` into before + 2.
+__global__ __launch_bounds__(64) void filter_moves_kernel(FilterParams F, uint32_t *moves, uint32_t *count,
+                                                          uint64_t *total) {
+    __shared__ uint8_t cidx[64];
+    const int lane = threadIdx.x;
+    const int L = F.cycle_len;
+    cidx[lane] = 0;
+    __syncthreads();
+    // sequential, so a repeated symbol keeps its last index as on the host
+    if (lane == 0)
+        for (int i = 0; i < L; ++i) cidx[F.cycle[i]] = (uint8_t)i;
+    __syncthreads();
+    auto dist = [&](int a, int b) {
+        const int ai = cidx[a], bi = cidx[b];
+        return bi >= ai ? bi - ai : (L - ai) + bi;
+    };
+    const int before = (L % 2 == 1) ? 1 : 0;
+    uint32_t cnt = 0;
+    for (int i = 0; i < L - 2; ++i) {
+        for (int j = i + 1; j < L - 1; ++j) {
+            const int k = j + 1 + lane;
+            bool ok = false;
+            uint32_t mv = 0;
+            if (k < L) {
+                const int a = F.cycle[i], b = F.cycle[j], c = F.cycle[k];
+                const int after = (dist(a, b) % 2 == 1) + (dist(b, c) % 2 == 1) + (dist(c, a) % 2 == 1);
+                ok = after - before == 2;
+                mv = (uint32_t)a | ((uint32_t)b << 8) | ((uint32_t)c << 16);
+            }
+            const unsigned long long m = __ballot(ok);
+            const unsigned long long below = (lane == 0) ? 0ULL : (m & (~0ULL >> (64 - lane)));
+            if (ok) moves[cnt + (uint32_t)__popcll(below)] = mv;
+            cnt += (uint32_t)__popcll(m);
+        }
+    }
+    if (lane == 0) {
+        *count = cnt;
+        *total = 0;
+    }
 }
 
 }  // namespace tdp
