@@ -11,9 +11,14 @@ Prints one JSON line: states expanded per second (value), cases per hour,
 the search kernel's roofline figure (algorithmic queue + dedup bytes of the
 launch, from the kernel's own counters, over its HIP-event duration on the
 slot's stream), the measured HBM bytes per launch when the rocprofv3 counter
-summary of this workload is present under profiles/, and the CPU baseline
-(the C restatement of the reference search, on a bounded sample of the same
-workload, over the host's cores).
+summary of this workload is present under profiles/, the kernel's vector-ALU
+issue rate against the chip's (VALU instructions per expansion from the
+committed SQ-counter summary times the live expansion rate), the learned
+ordering timed on the same resident batch (``learned``: states/s, cases/h
+and expansions to verdict on the satisfiable configurations beside the plain
+search's), and the CPU baseline (the C restatement of the reference search,
+on a bounded sample of the same workload, over the host's cores, whose
+decided verdicts must equal the GPU's).
 
 ``--copy-calibration MB`` instead runs a plain device copy of MB MiB (the
 known-size kernel that scales the FETCH_SIZE / WRITE_SIZE counters).
@@ -30,6 +35,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# Vector-ALU issue peak of MI355X in wave64 instructions per second: 256 CUs x
+# 4 SIMDs, one wave64 VALU instruction per SIMD every 4 cycles (16 lanes per
+# cycle), at the 2.4 GHz peak engine clock.
+VALU_PEAK_INSTS = 256 * 4 * 2.4e9 / 4
 METRIC = "states expanded/s + cases/hour at depth 11, 1 MI355X; learned vs plain ordering"
 
 
@@ -61,6 +70,16 @@ def measured_traffic(workload: str):
         return json.load(fh).get("hbm_bytes_per_dispatch")
 
 
+def sq_profile(workload: str):
+    """Per-expansion instruction counts of search_kernel from the committed
+    rocprofv3 SQ-counter summary of this workload, or None when absent."""
+    path = os.path.join(ROOT, "profiles", f"search_kernel_{workload}_sq_counters.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        return json.load(fh)
+
+
 def copy_calibration(mb: int) -> None:
     import torch
     n = (mb << 20) // 4
@@ -72,16 +91,63 @@ def copy_calibration(mb: int) -> None:
     print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4, "dispatches": 3}))
 
 
-def cpu_baseline(wl, sample_cases: int, cap: int, threads: int):
+def cpu_baseline(wl, sample_cases: int, cap: int, threads: int, gpu_found):
+    """The oracle's search (exact dedup, one queue, children evaluated by arc
+    re-linking) over the first sample_cases configurations, one configuration
+    per host thread.  Every verdict it reaches within the cap must equal the
+    GPU's."""
     import oracle
     cases = wl.cases[:sample_cases]
     res, secs = oracle.search_many([c.pi for c in cases], [c.spi for c in cases], [c.n for c in cases],
                                    [c.even_cycles for c in cases], [wl.min_rate] * len(cases),
-                                   [wl.max_depth] * len(cases), [0] * len(cases), True, cap, threads)
+                                   [wl.max_depth] * len(cases), [0] * len(cases), True, cap, threads, arcs=True)
+    differ = [c for c, r in enumerate(res) if not r.aborted and r.found != bool(gpu_found[c])]
+    if differ:
+        raise RuntimeError(f"CPU baseline verdicts differ from the GPU's on configurations {differ[:10]}")
     exps = sum(r.counters["expansions"] for r in res)
+    decided = sum(not r.aborted for r in res)
     return {"value": exps / secs, "unit": "states/s", "cores": threads, "kind": "port",
             "sample": f"first {len(cases)} configurations of the same batch, each capped at {cap} expansions "
-                      f"({exps} expansions in {secs:.1f} s, {sum(r.aborted for r in res)} capped)"}
+                      f"({exps} expansions in {secs:.1f} s, {len(cases) - decided} capped); the {decided} "
+                      f"verdicts reached equal the GPU's",
+            "cases_per_hour_decided": decided / secs * 3600.0}
+
+
+def learned_leg(T, wl, step, verdicts, args, plain_exps_sat):
+    """The learned ordering (TDP_FLAG_LEARNED) over the same resident batch:
+    one untimed step, then args.learned_steps timed steps; verdicts must equal
+    the plain search's."""
+    import numpy as np
+    from tdp1375_amd import train_ordering
+    train_ordering.load(args.model)
+    flags = T._lib.FLAG_LEARNED | (T._lib.FLAG_GRAPH if args.graph else 0)
+    r = step(resident=True, extra=flags)
+    if not np.array_equal(r.found, verdicts):
+        raise RuntimeError("learned ordering changed a verdict")
+    exps, kernel_ms, sat = 0, [], []
+    t0 = time.perf_counter()
+    for s in range(args.learned_steps):
+        r = step(resident=True, extra=flags)
+        if not np.array_equal(r.found, verdicts):
+            raise RuntimeError("learned ordering changed a verdict")
+        exps += int(r.counters["expansions"].sum())
+        kernel_ms.append(T.last_launch_stats(0)["kernel_ms"])
+        sat.append([int(x) for x in r.counters["expansions"][verdicts]])
+        print(f"learned step {s}: {kernel_ms[-1]:.1f} ms, {int(r.counters['expansions'].sum())} expansions",
+              file=sys.stderr, flush=True)
+    wall = time.perf_counter() - t0
+    lsat = np.array(sat[-1], np.float64)
+    psat = np.array(plain_exps_sat, np.float64)
+    return {"value": exps / wall, "unit": "states/s", "steps": args.learned_steps,
+            "ms_per_step": 1e3 * wall / args.learned_steps,
+            "cases_per_hour": len(wl.cases) * args.learned_steps / wall * 3600.0,
+            "verdicts_equal_plain": True,
+            "satisfiable": int(verdicts.sum()),
+            "expansions_to_verdict_satisfiable": {"learned": float(lsat.sum()), "plain": float(psat.sum()),
+                                                  "plain_over_learned": float(psat.sum() / max(lsat.sum(), 1.0)),
+                                                  "median_learned": float(np.median(lsat)),
+                                                  "median_plain": float(np.median(psat))},
+            "model": os.path.relpath(args.model, ROOT)}
 
 
 def main():
@@ -93,8 +159,10 @@ def main():
     ap.add_argument("--queue-mb", type=int, default=32768)
     ap.add_argument("--dedup-log2", type=int, default=27)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-sample", type=int, default=192)
-    ap.add_argument("--cpu-cap", type=int, default=40000)
+    ap.add_argument("--cpu-sample", type=int, default=256)
+    ap.add_argument("--cpu-cap", type=int, default=100000)
+    ap.add_argument("--learned-steps", type=int, default=2,
+                    help="timed steps of the learned ordering on the same batch (0: skip)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB")
     ap.add_argument("--graph", action="store_true",
                     help="run each launch round as the slot's HIP graph (TDP_FLAG_GRAPH)")
@@ -141,15 +209,17 @@ def main():
     if args.graph:
         order_flag |= T._lib.FLAG_GRAPH
 
-    def step(resident: bool):
+    def step(resident: bool, extra=None):
         return T.search_batch(0, pis, spis, e0s, wl.min_rate, wl.max_depth, 0,
-                              flags=(T._lib.FLAG_RESIDENT if resident else 0) | order_flag)
+                              flags=(T._lib.FLAG_RESIDENT if resident else 0) |
+                              (order_flag if extra is None else extra))
 
     first = None
     for w in range(max(1, args.warmup)):
         first = step(resident=w > 0)
         print(f"warmup {w}: found {int(first.found.sum())}/{len(cases)} "
-              f"{T.last_launch_stats(0)['kernel_ms']:.1f} ms", file=sys.stderr, flush=True)
+              f"{T.last_launch_stats(0)['kernel_ms']:.1f} ms, {int(first.counters['expansions'].sum())} expansions",
+              file=sys.stderr, flush=True)
     verdicts = first.found.copy()
 
     # every search_batch call returns only after its stream has drained, so
@@ -170,6 +240,11 @@ def main():
         print(f"step {s}: {st['kernel_ms']:.1f} ms, {int(r.counters['expansions'].sum())} expansions",
               file=sys.stderr, flush=True)
     wall = time.perf_counter() - t0
+    plain_exps_sat = [int(x) for x in r.counters["expansions"][verdicts]]
+
+    learned = None
+    if args.learned_steps > 0 and args.ordering == "plain":
+        learned = learned_leg(T, wl, step, verdicts, args, plain_exps_sat)
     T.destroy_slots()
 
     ms_per_step = 1e3 * wall / args.steps
@@ -182,6 +257,18 @@ def main():
     bytes_per_dispatch = sum(traffic) / len(traffic) / dispatches
     achieved = bytes_per_dispatch / avg_kernel_s / 1e9
     cases_per_hour = len(cases) * args.steps / wall * 3600.0
+    # the graph's events bracket the whole round (reset kernel, search kernel,
+    # control copy), so in graph mode the time is a round's, not the kernel's
+    kernel_label = "search_graph_round" if args.graph else \
+        ("search_learned_kernel" if args.ordering == "learned" else "search_kernel")
+    sq = sq_profile(wl.name) if args.ordering == "plain" else None
+    valu = None
+    if sq:
+        kernel_s = 1e-3 * sum(kernel_ms)
+        valu_rate = sq["valu_insts_per_expansion"] * exps / kernel_s
+        valu = {"valu_insts_per_expansion": sq["valu_insts_per_expansion"],
+                "achieved_insts_per_s": valu_rate, "peak_insts_per_s": VALU_PEAK_INSTS,
+                "valu_frac": valu_rate / VALU_PEAK_INSTS}
     out = {
         "metric": METRIC,
         "value": exps / wall,
@@ -203,13 +290,16 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": measured_traffic(wl.name) if args.ordering == "plain" else None,
-                     "kernel": "search_kernel", "dispatches_per_step": dispatches,
+                     "kernel": kernel_label, "dispatches_per_step": dispatches,
                      "avg_kernel_ms": 1e3 * avg_kernel_s,
-                     "algorithmic_bytes_per_launch": bytes_per_dispatch},
+                     "algorithmic_bytes_per_launch": bytes_per_dispatch,
+                     "valu_frac": valu["valu_frac"] if valu else None, "valu": valu},
     }
+    if learned is not None:
+        out["learned"] = learned
     if not args.no_cpu_baseline:
         threads = max(1, min(16, os.cpu_count() or 1))
-        out["cpu_baseline"] = cpu_baseline(wl, args.cpu_sample, args.cpu_cap, threads)
+        out["cpu_baseline"] = cpu_baseline(wl, args.cpu_sample, args.cpu_cap, threads, verdicts)
     print(json.dumps(out))
 
 
