@@ -135,6 +135,46 @@ TDP_API int tdp_search_batch(int slot, int count, const uint8_t *pis, const uint
                      const int32_t *max_moves, const int32_t *full_sorting, uint32_t flags,
                      tdp_case_result *out_results);
 
+/* ---- device-resident serving (extension; replaces the one-launch-per-call
+ * path of performGpuSortingSearch, sorting_search.cu:701-856, for the
+ * ForkJoin callers of GPUSortingSearch.search, GPUSortingSearch.java:47-79,
+ * SortOrExtend.java:96-104).  tdp_serve_start launches one persistent search
+ * on the current device that runs until tdp_serve_stop (or lifetime_s
+ * seconds): its waves take requests as they arrive, share their work, the
+ * queue budget and a dedup table of dedup_table_size entries, and reply
+ * through pinned host memory.  While a session runs, tdp_search (and so
+ * searchForSortingSeq) on a slot of that device is served by it, and
+ * tdp_cancel cancels the slot's running request.  A request is admitted when
+ * one of `slots` (<= 65536) device slots is free and the roots of its start
+ * node (C(n,3) nodes) fit beside those of the requests in flight within half
+ * of the spill pool (a quarter of queue_bytes); slots bounds the requests in
+ * flight and so their demand on the shared dedup table; max_n (<= 63) bounds
+ * n and sets the node size.  flags:
+ * TDP_FLAG_LEARNED orders children by the model set at start.  Verdicts are
+ * the plain search's; counters in replies are those of the batches ended
+ * before the verdict.  initSlots starts a session itself when TDP_SERVE=1 is
+ * in the environment, with the slots' total queue budget and dedup entries
+ * and TDP_SERVE_SLOTS (default 256) requests in flight; the slots then hold
+ * no queue memory of their own. */
+TDP_API int tdp_serve_start(int64_t queue_bytes, int dedup_table_size, int slots, int max_n, uint32_t flags,
+                            int lifetime_s);
+TDP_API int tdp_serve_stop(void);
+/* One request through the running session; blocks until its verdict.  Same
+ * arguments and results as tdp_search; thread-safe. */
+TDP_API int tdp_serve_search(const uint8_t *pi, const uint8_t *spi, int n, int initial_even_cycles,
+                             float min_rate, int max_moves, int full_sorting, int32_t *out_moves,
+                             int32_t *out_depth, tdp_counters *out_counters);
+/* A batch through the running session from one caller (same layout and
+ * results as tdp_search_batch): requests are admitted as slots and queue
+ * memory free up while the earlier ones run, and the call returns when every
+ * verdict is in. */
+TDP_API int tdp_serve_batch(int count, const uint8_t *pis, const uint8_t *spis, const int32_t *ns,
+                            const int32_t *initial_even_cycles, const float *min_rates, const int32_t *max_moves,
+                            const int32_t *full_sorting, tdp_case_result *out_results);
+/* Requests completed (slot handed back), slots in use and queue bytes
+ * reserved by the requests in flight. */
+TDP_API int tdp_serve_stats(uint64_t *completed, uint64_t *in_flight, int64_t *reserved_bytes);
+
 /* Statistics of the last launch on a slot. */
 TDP_API int tdp_last_launch_stats(int slot, tdp_launch_stats *out);
 

@@ -37,6 +37,7 @@ EXPORTED_SYMBOLS = (
     "tdp_host_predicate_tables", "tdp_set_tuning", "tdp_set_launch_deadline", "tdp_set_expansion_batch",
     "tdp_canonical_batch",
     "tdp_set_alloc_timeout", "tdp_alloc_retries",
+    "tdp_serve_start", "tdp_serve_stop", "tdp_serve_search", "tdp_serve_batch", "tdp_serve_stats",
     "tdp_device_count",
     "tdp_mlp_set_weights",
     "tdp_mlp_get_weights", "tdp_mlp_init", "tdp_mlp_forward", "tdp_mlp_gradients", "tdp_mlp_train",
@@ -45,6 +46,8 @@ EXPORTED_SYMBOLS = (
     "Java_br_unb_cic_tdp_base_GPUSortingSearch_searchForSortingSeq",
     "Java_br_unb_cic_tdp_base_GPUSortingSearch_cancelSort",
     "Java_br_unb_cic_tdp_base_GPUSortingSearch_filterPiPermutations",
+    "Java_br_unb_cic_tdp_base_GPUSortingSearch_searchBatch",
+    "Java_br_unb_cic_tdp_base_GPUSortingSearch_canonicalBatch",
 )
 
 
@@ -125,6 +128,15 @@ def load() -> ctypes.CDLL:
         L.tdp_filter_noalloc.argtypes = [u8p, ctypes.c_int, u8p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                          ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p]
         L.tdp_set_alloc_timeout.argtypes = [ctypes.c_int64]
+        L.tdp_serve_start.argtypes = [ctypes.c_int64, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_uint32,
+                                      ctypes.c_int]
+        L.tdp_serve_stop.argtypes = []
+        L.tdp_serve_search.argtypes = [u8p, u8p, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_int,
+                                       ctypes.c_int, i32p, i32p, ctypes.POINTER(Counters)]
+        L.tdp_serve_batch.argtypes = [ctypes.c_int, u8p, u8p, i32p, i32p, ctypes.POINTER(ctypes.c_float), i32p, i32p,
+                                      ctypes.POINTER(CaseResult)]
+        L.tdp_serve_stats.argtypes = [ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64),
+                                      ctypes.POINTER(ctypes.c_int64)]
         L.tdp_alloc_retries.argtypes = []
         L.tdp_alloc_retries.restype = ctypes.c_uint64
         L.tdp_eval_children.argtypes = [u8p, u8p, ctypes.c_int, i32p]

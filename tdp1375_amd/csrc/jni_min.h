@@ -20,6 +20,7 @@ typedef jobject jthrowable;
 typedef jobject jarray;
 typedef jarray jbyteArray;
 typedef jarray jintArray;
+typedef jarray jfloatArray;
 typedef jarray jobjectArray;
 
 #define JNI_ABORT 2
@@ -47,8 +48,10 @@ enum JniIndex {
     kJniNewByteArray = 176,
     kJniNewIntArray = 179,
     kJniGetByteArrayRegion = 200,
+    kJniGetIntArrayRegion = 203,
     kJniSetByteArrayRegion = 208,
     kJniSetIntArrayRegion = 211,
+    kJniSetFloatArrayRegion = 213,
     kJniExceptionCheck = 228,
 };
 
@@ -91,6 +94,13 @@ inline void GetByteArrayRegion(JNIEnv *env, jbyteArray a, jsize start, jsize len
 inline void SetByteArrayRegion(JNIEnv *env, jbyteArray a, jsize start, jsize len, const jbyte *buf) {
     fn<void (*)(JNIEnv *, jbyteArray, jsize, jsize, const jbyte *)>(env, kJniSetByteArrayRegion)(env, a, start, len,
                                                                                               buf);
+}
+inline void GetIntArrayRegion(JNIEnv *env, jintArray a, jsize start, jsize len, jint *buf) {
+    fn<void (*)(JNIEnv *, jintArray, jsize, jsize, jint *)>(env, kJniGetIntArrayRegion)(env, a, start, len, buf);
+}
+inline void SetFloatArrayRegion(JNIEnv *env, jfloatArray a, jsize start, jsize len, const jfloat *buf) {
+    fn<void (*)(JNIEnv *, jfloatArray, jsize, jsize, const jfloat *)>(env, kJniSetFloatArrayRegion)(env, a, start,
+                                                                                                  len, buf);
 }
 inline void SetIntArrayRegion(JNIEnv *env, jintArray a, jsize start, jsize len, const jint *buf) {
     fn<void (*)(JNIEnv *, jintArray, jsize, jsize, const jint *)>(env, kJniSetIntArrayRegion)(env, a, start, len, buf);

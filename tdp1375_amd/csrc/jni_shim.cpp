@@ -4,7 +4,9 @@
 // src/native/jni/jni_sorter.cu:16-150: inputs are copied, bytes are read
 // unsigned, results are fresh Java arrays, and every failure becomes a
 // java.lang.RuntimeException carrying the native message while the call
-// returns null.  cancelSort never throws.
+// returns null.  cancelSort never throws.  Two more natives (no reference
+// counterpart) expose the batched search and the batched canonical forms
+// to the same class with the same conventions.
 #include <stdint.h>
 
 #include <string>
@@ -23,6 +25,13 @@ void throw_runtime(JNIEnv *env, const char *message) {
         jni::ThrowNew(env, cls, message);
         jni::DeleteLocalRef(env, cls);
     }
+}
+
+bool read_ints(JNIEnv *env, jintArray arr, jsize len, std::vector<int32_t> &out) {
+    out.assign((size_t)(len > 0 ? len : 0), 0);
+    if (len <= 0) return true;
+    jni::GetIntArrayRegion(env, arr, 0, len, reinterpret_cast<jint *>(out.data()));
+    return !jni::ExceptionCheck(env);
 }
 
 bool read_bytes(JNIEnv *env, jbyteArray arr, jsize len, std::vector<uint8_t> &out) {
@@ -111,6 +120,89 @@ JNIEXPORT jobjectArray JNICALL Java_br_unb_cic_tdp_base_GPUSortingSearch_filterP
     }
     jni::DeleteLocalRef(env, byteArrayClass);
     return outer;
+}
+
+// Many configurations in one persistent launch (tdp_search_batch).  Java:
+//   private static native int[] searchBatch(byte[] pis, byte[] spis, int[] ns, int[] initialEvenCycles,
+//                                           float minRate, int maxMoves, boolean fullSorting, int flags,
+//                                           int slot);
+// pis/spis hold count x 64 bytes (configuration c in bytes 64c .. 64c + ns[c]);
+// the result holds per configuration found, depth and 15 position triples
+// (47 ints), a depth of -1 marking a configuration left undecided by
+// TDP_FLAG_SOFT_DEADLINE.
+JNIEXPORT jintArray JNICALL Java_br_unb_cic_tdp_base_GPUSortingSearch_searchBatch(
+    JNIEnv *env, jclass, jbyteArray javaPis, jbyteArray javaSpis, jintArray javaNs, jintArray javaE0s, jfloat minRate,
+    jint maxMoves, jboolean fullSorting, jint flags, jint slot) {
+    const jsize count = jni::GetArrayLength(env, javaNs);
+    if (jni::GetArrayLength(env, javaE0s) < count || jni::GetArrayLength(env, javaPis) < 64 * count ||
+        jni::GetArrayLength(env, javaSpis) < 64 * count) {
+        throw_runtime(env, "performGpuSortingSearch: inconsistent input sizes");
+        return nullptr;
+    }
+    std::vector<uint8_t> pis, spis;
+    std::vector<int32_t> ns, e0s;
+    if (!read_bytes(env, javaPis, 64 * count, pis) || !read_bytes(env, javaSpis, 64 * count, spis) ||
+        !read_ints(env, javaNs, count, ns) || !read_ints(env, javaE0s, count, e0s))
+        return nullptr;
+    std::vector<float> rates((size_t)count, minRate);
+    std::vector<int32_t> depths((size_t)count, maxMoves), fulls((size_t)count, fullSorting ? 1 : 0);
+    std::vector<tdp_case_result> res((size_t)count);
+    if (tdp_search_batch(slot, count, pis.data(), spis.data(), ns.data(), e0s.data(), rates.data(), depths.data(),
+                         fulls.data(), (uint32_t)flags, res.data()) != TDP_OK) {
+        throw_runtime(env, tdp_last_error());
+        return nullptr;
+    }
+    constexpr int kPer = 2 + 3 * TDP_MAX_DEPTH;
+    std::vector<jint> out((size_t)count * kPer, 0);
+    for (jsize c = 0; c < count; ++c) {
+        jint *o = out.data() + (size_t)c * kPer;
+        o[0] = res[c].found;
+        o[1] = res[c].depth;
+        for (int m = 0; m < res[c].depth && m < TDP_MAX_DEPTH; ++m)
+            for (int q = 0; q < 3; ++q) o[2 + 3 * m + q] = res[c].moves[m][q];
+    }
+    jintArray arr = jni::NewIntArray(env, (jsize)out.size());
+    jni::SetIntArrayRegion(env, arr, 0, (jsize)out.size(), out.data());
+    return arr;
+}
+
+// Canonical forms of many configurations in one launch (tdp_canonical_batch,
+// for Configuration.getCanonical / hashCode, Configuration.java:156-177).
+// Java:
+//   private static native int canonicalBatch(byte[] spis, byte[] pis, int[] ns, float[] signatures,
+//                                            int[] hashes, int[] index, byte[] canonicalSpi);
+// spis/pis/canonicalSpi hold count x 64 bytes, signatures count x 64 floats;
+// returns count.
+JNIEXPORT jint JNICALL Java_br_unb_cic_tdp_base_GPUSortingSearch_canonicalBatch(
+    JNIEnv *env, jclass, jbyteArray javaSpis, jbyteArray javaPis, jintArray javaNs, jfloatArray javaSignatures,
+    jintArray javaHashes, jintArray javaIndex, jbyteArray javaCanonical) {
+    const jsize count = jni::GetArrayLength(env, javaNs);
+    if (jni::GetArrayLength(env, javaSpis) < 64 * count || jni::GetArrayLength(env, javaPis) < 64 * count ||
+        jni::GetArrayLength(env, javaSignatures) < 64 * count || jni::GetArrayLength(env, javaHashes) < count ||
+        jni::GetArrayLength(env, javaIndex) < count || jni::GetArrayLength(env, javaCanonical) < 64 * count) {
+        throw_runtime(env, "canonicalBatch: inconsistent array sizes");
+        return 0;
+    }
+    std::vector<uint8_t> spis, pis;
+    std::vector<int32_t> ns;
+    if (!read_bytes(env, javaSpis, 64 * count, spis) || !read_bytes(env, javaPis, 64 * count, pis) ||
+        !read_ints(env, javaNs, count, ns))
+        return 0;
+    std::vector<float> sig((size_t)count * 64);
+    std::vector<int32_t> hashes((size_t)count), index((size_t)count);
+    std::vector<uint8_t> canon((size_t)count * 64);
+    if (count > 0 && tdp_canonical_batch(count, spis.data(), pis.data(), ns.data(), sig.data(), hashes.data(),
+                                         index.data(), canon.data()) != TDP_OK) {
+        throw_runtime(env, tdp_last_error());
+        return 0;
+    }
+    if (count > 0) {
+        jni::SetFloatArrayRegion(env, javaSignatures, 0, 64 * count, sig.data());
+        jni::SetIntArrayRegion(env, javaHashes, 0, count, hashes.data());
+        jni::SetIntArrayRegion(env, javaIndex, 0, count, index.data());
+        jni::SetByteArrayRegion(env, javaCanonical, 0, 64 * count, reinterpret_cast<const jbyte *>(canon.data()));
+    }
+    return count;
 }
 
 }  // extern "C"

@@ -166,6 +166,7 @@ enum ControlIndex {
     kCtlEvictions,     // dedup entries replaced while live (bucket full)
     kCtlSpillTakes,    // nodes taken from the spill pool
     kCtlOverflowInfo,  // why the overflow flag was raised: reason << 56 | stack fill << 32 | pool fill
+    kCtlPublished,     // serving: requests copied to device slots by the feeder
     kCtlPhase0,        // per-phase time of profiling builds (kNumPhases lines)
     kNumControl = kCtlPhase0 + 8
 };
@@ -182,5 +183,44 @@ enum PhaseIndex {
     kPhaseRoot,         // root expansions
     kNumPhases
 };
+
+// ---- device-resident serving (serve_kernel)
+// A request as the host publishes it in its pinned request ring: the search
+// descriptor, the device slot it is to run in (chosen by the host among the
+// drained slots), and seq = request index + 1 written last.
+struct alignas(16) ServeRequest {
+    ConfigDesc desc;
+    uint32_t slot;
+    uint32_t pad0;
+    unsigned long long seq;
+};
+
+// The reply of a slot's current request in pinned host memory.  seq =
+// request index + 1 once the verdict is in (found: as soon as a wave finds
+// it; not found / cancelled: once the request's last node is gone), drained
+// = request index + 1 once no node of the request is left anywhere, after
+// which the host may give the slot to another request.
+struct alignas(16) ServeReply {
+    int32_t found;           // 1 found, 0 not found, 2 cancelled
+    int32_t depth;
+    uint32_t moves[16];      // triple indices of the sequence, in move order
+    unsigned long long counters[kNumCounters];   // so far (found) / final (not found)
+    unsigned long long seq;
+    unsigned long long drained;
+};
+
+// Host control words of a serving session (pinned host memory).
+enum ServeHostIndex {
+    kServeHostStop = 0,      // set by the host: the session ends
+    kServeHostConsumed = 1,  // written by the feeder: requests copied so far
+    kServeHostCancels = 2,   // set by the host: cancellations published so far
+    kServeHostCancelRing = 16,   // kServeCancelEntries words: request << 16 | slot
+    kServeHostWords = 16 + 256
+};
+constexpr int kServeCancelEntries = 256;
+
+// slot_state word of a serving slot: generation << 2 | done (bit 0: the
+// request is found, cancelled or drained, so its dedup entries may go).
+constexpr unsigned long long kSlotDone = 1ULL;
 
 }  // namespace tdp

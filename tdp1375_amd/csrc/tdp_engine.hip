@@ -13,6 +13,7 @@
 #include <atomic>
 #include <chrono>
 #include <cmath>
+#include <condition_variable>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -440,6 +441,7 @@ struct Slot {
     DevPtr cancel_flag;        // int on the device, polled by the kernel
     HostPtr cancel_one;        // pinned int holding 1 (slot_manager.cu:86-87)
     std::atomic<bool> cancelled{false};
+    std::atomic<int64_t> serve_ticket{-1};   // serving: request << 16 | slot of the running search
     std::mutex run_mutex;      // one search at a time per slot
 
     int64_t queue_budget = 0;
@@ -519,11 +521,24 @@ void destroy_all_slots() {
     if (cur >= 0) (void)hipSetDevice(cur);
 }
 
+void serve_stop();
+void serve_start(int64_t queue_bytes, int64_t dedup, int slots, int max_n, uint32_t flags, int lifetime_s);
+
+// TDP_SERVE=1 in the environment of initSlots: the slots become handles of
+// one serving session on device 0 that holds all of their queue memory, so
+// concurrent searchForSortingSeq calls share one persistent launch.
+bool serving_requested() {
+    const char *e = std::getenv("TDP_SERVE");
+    return e && std::atoi(e) != 0;
+}
+
 void init_slots(int total, int devices, int64_t budget, int64_t dedup) {
     if (total <= 0) throw std::invalid_argument("totalSlots must be positive");
     if (devices <= 0) throw std::invalid_argument("devicesCount must be positive");
     if (budget < 0) throw std::invalid_argument("queueBytesBudget must be non-negative");
+    serve_stop();
     destroy_all_slots();
+    const bool serving = serving_requested();
     int count = 0;
     HIP_CHECK(hipGetDeviceCount(&count));
     std::lock_guard<std::mutex> lk(g_slots_mutex);
@@ -546,10 +561,10 @@ void init_slots(int total, int devices, int64_t budget, int64_t dedup) {
         HIP_CHECK(hipMemset(s->cancel_flag.get(), 0, sizeof(int)));
         s->cancel_one = make_pinned(sizeof(int));
         *as<int>(s->cancel_one) = 1;
-        s->queue_budget = budget;
-        s->dedup_size = std::max<int64_t>(0, dedup);
+        s->queue_budget = serving ? 0 : budget;
+        s->dedup_size = serving ? 0 : std::max<int64_t>(0, dedup);
         (void)device_shared(s->device);
-        s->queue_bytes = (size_t)budget;
+        s->queue_bytes = serving ? 0 : (size_t)budget;
         s->queue_mem = make_dev(std::max<size_t>(s->queue_bytes, 256), "sort.nodes");
         s->ctrl = make_dev(sizeof(ControlLine) * kNumControl, "sort.control");
         s->h_ctrl = make_pinned(sizeof(ControlLine) * kNumControl);
@@ -566,6 +581,16 @@ void init_slots(int total, int devices, int64_t budget, int64_t dedup) {
         }
         s->epoch = 0;
         g_slots.push_back(std::move(s));
+    }
+    if (serving) {
+        // the session holds what the slots would have held: their queue
+        // budgets and their dedup tables (each reference slot has its own)
+        int lifetime = 86400, inflight = 256;
+        if (const char *e = std::getenv("TDP_SERVE_LIFETIME_S")) lifetime = std::max(1, std::atoi(e));
+        if (const char *e = std::getenv("TDP_SERVE_SLOTS")) inflight = std::max(1, std::min(65536, std::atoi(e)));
+        HIP_CHECK(hipSetDevice(0));
+        serve_start(budget * (int64_t)total, std::max<int64_t>(0, dedup) * (int64_t)total, inflight, kMaxN, 0u,
+                    lifetime);
     }
     } catch (...) {
         if (s) destroy_slot(*s);
@@ -921,6 +946,450 @@ void clear_cancel(Slot &s) {
     HIP_CHECK(hipMemsetAsync(s.cancel_flag.get(), 0, sizeof(int), s.stream));
 }
 
+// ---------------------------------------------------------------- serving
+// A device-resident request loop (serve_kernel): one persistent launch whose
+// waves take requests as they arrive, share their work and the session's
+// queue memory and dedup table, and reply through pinned host memory.  Host
+// threads submit requests concurrently (tdp_serve_search blocks until its
+// reply), and tdp_search routes to the session while one runs, so
+// concurrent searchForSortingSeq calls of ForkJoin workers share one launch.
+// Admission: a request waits for a free device slot, a free request-ring
+// entry and, unless nothing else is in flight, for room in the queue memory:
+// the roots of its start node (C(n,3) nodes, all pushed by the wave that
+// takes it, spilling to the shared pool when they do not fit) must fit beside
+// the roots of the requests in flight within half of the spill pool.  The
+// number of slots bounds the requests in flight, and with them the demand on
+// the shared dedup table (a table too full for the requests in flight admits
+// children without entries, which costs repeated work).
+struct ServeSession {
+    int device = -1;
+    hipStream_t stream = nullptr;
+    bool learned = false;
+    int slots = 0;
+    uint32_t ring = 0;
+    int max_n = 0;
+    size_t node_bytes = 0;
+    int64_t reserve_limit = 0, reserved = 0;
+    DevPtr queue_mem, share_lines, pool_seq, ctrl, dd_tag, dd_key, cfgs, results, slot_state, req_index,
+        ring_slot;
+    HostPtr host_req, host_reply, host_ctl;
+    ServeRequest *h_req = nullptr;
+    ServeReply *h_reply = nullptr;
+    unsigned long long *h_ctl = nullptr;
+    SearchParams P{};
+    std::mutex mu;
+    std::condition_variable cv;
+    uint64_t next_request = 0;
+    uint64_t cancels = 0;
+    std::vector<int> free_slots;
+    std::vector<uint64_t> slot_request;   // request in the slot, ~0 when free
+    std::vector<int64_t> slot_need;
+    std::vector<uint8_t> slot_replied;    // the submitting thread has read the reply
+    std::vector<uint32_t> slot_uses;
+    bool ended = false;                   // the launch is over (stop, overflow, watchdog)
+    unsigned long long end_reason = 0;
+    uint64_t completed = 0;
+};
+std::mutex g_serve_mutex;                 // start / stop
+std::shared_ptr<ServeSession> g_serve;    // the running session, if any
+
+std::shared_ptr<ServeSession> current_serve() {
+    std::lock_guard<std::mutex> lk(g_serve_mutex);
+    return g_serve;
+}
+
+template <typename T>
+T load_acquire(const T *p) { return __atomic_load_n(p, __ATOMIC_ACQUIRE); }
+template <typename T>
+void store_release(T *p, T v) { __atomic_store_n(p, v, __ATOMIC_RELEASE); }
+
+HostPtr make_pinned_coherent(size_t bytes) {
+    void *p = nullptr;
+    HIP_CHECK(hipHostMalloc(&p, bytes, hipHostMallocCoherent | hipHostMallocMapped));
+    std::memset(p, 0, bytes);
+    return HostPtr(p);
+}
+
+template <typename T>
+T *device_view(const HostPtr &h) {
+    void *d = nullptr;
+    HIP_CHECK(hipHostGetDevicePointer(&d, h.get(), 0));
+    return static_cast<T *>(d);
+}
+
+// The launch has ended when its stream is idle; records why (stop word).
+bool serve_ended(ServeSession &S) {
+    if (S.ended) return true;
+    const hipError_t st = hipStreamQuery(S.stream);
+    if (st == hipErrorNotReady) return false;
+    (void)hipGetLastError();
+    unsigned long long why = 0;
+    (void)hipMemcpy(&why, &as<ControlLine>(S.ctrl)[kCtlStop].v, sizeof(why), hipMemcpyDeviceToHost);
+    S.end_reason = why;
+    S.ended = true;
+    return true;
+}
+
+[[noreturn]] void serve_fail(const ServeSession &S) {
+    if (S.end_reason == 1) throw QueueOverflow("GPU sorting search queue overflow");
+    if (S.end_reason == 2) throw std::runtime_error("GPU sorting search watchdog timeout (serving session lifetime)");
+    throw std::runtime_error("serving session stopped");
+}
+
+// Slots whose request has drained and whose reply was read go back to the
+// free list (caller holds S.mu).
+void serve_reclaim(ServeSession &S) {
+    for (int s = 0; s < S.slots; ++s) {
+        const uint64_t r = S.slot_request[s];
+        if (r == ~0ULL || !S.slot_replied[s]) continue;
+        if (load_acquire(&S.h_reply[s].drained) != r + 1) continue;
+        S.slot_request[s] = ~0ULL;
+        S.reserved -= S.slot_need[s];
+        S.slot_need[s] = 0;
+        S.free_slots.push_back(s);
+        S.completed++;
+    }
+}
+
+void serve_start(int64_t queue_bytes, int64_t dedup, int slots, int max_n, uint32_t flags, int lifetime_s) {
+    if (slots < 1 || slots > kMaxBatchConfigs) throw std::invalid_argument("serving slots must be in 1..65536");
+    if (max_n < 3 || max_n > kMaxN) throw std::invalid_argument("serving max_n must be in 3..63");
+    if (queue_bytes <= 0) throw std::invalid_argument("queueBytesBudget must be positive");
+    if (lifetime_s <= 0) throw std::invalid_argument("serving lifetime must be positive");
+    std::lock_guard<std::mutex> lk(g_serve_mutex);
+    if (g_serve) throw std::invalid_argument("a serving session is already running");
+    auto S = std::make_shared<ServeSession>();
+    HIP_CHECK(hipGetDevice(&S->device));
+    DeviceShared &dev = device_shared(S->device);
+    S->learned = (flags & TDP_FLAG_LEARNED) != 0;
+    S->slots = slots;
+    S->ring = (uint32_t)slots;
+    S->max_n = max_n;
+    const uint32_t nw = (uint32_t)node_words_for(max_n);
+    S->node_bytes = (size_t)nw * 4;
+    const Tuning tune = current_tuning();
+    // the feeder takes the first block; the search waves the rest (all of
+    // them resident: the grid is the serving kernel's occupancy limit)
+    int occ_blocks = 0;
+    HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &occ_blocks, S->learned ? serve_learned_kernel : serve_kernel, 256, 0));
+    int waves = std::max(1, occ_blocks) * dev.cus * 4 - 4;
+    if (tune.waves_per_cu > 0) waves = std::min(waves, std::max(4, tune.waves_per_cu * dev.cus));
+    waves = std::max(4, waves - waves % 4);
+    uint32_t share_max = (uint32_t)tune.share_max;
+    while (share_max > 1 && (size_t)waves * share_max * S->node_bytes * 4 > (size_t)queue_bytes) share_max /= 2;
+    const size_t inbox_bytes = (size_t)waves * share_max * S->node_bytes;
+    if (inbox_bytes * 4 > (size_t)queue_bytes) throw QueueOverflow("GPU sorting search queue overflow");
+    const size_t pool_bytes = ((size_t)queue_bytes - inbox_bytes) / 4;
+    const size_t stack_bytes = (size_t)queue_bytes - inbox_bytes - pool_bytes;
+    const uint32_t stack_cap = floor_pow2(stack_bytes / ((size_t)waves * S->node_bytes));
+    if (stack_cap < 2) throw QueueOverflow("GPU sorting search queue overflow");
+    if (share_max > stack_cap / 2) share_max = stack_cap / 2;
+    uint32_t pool_cap = floor_pow2(pool_bytes / S->node_bytes);
+    if (pool_cap < 64) pool_cap = 0;
+    // the roots of a request go onto the stack of the wave that takes it and
+    // spill to the pool when they do not fit; the roots of the requests in
+    // flight are kept within half of the pool
+    S->reserve_limit = (int64_t)std::max<size_t>(pool_bytes / 2, S->node_bytes);
+
+    S->queue_mem = make_dev((size_t)queue_bytes, "serve.nodes");
+    const uint32_t idle_words = (uint32_t)((waves + 63) / 64);
+    S->share_lines = make_dev(((size_t)waves + idle_words) * 128, "serve.shareLines");
+    if (pool_cap) S->pool_seq = make_dev((size_t)pool_cap * 8, "serve.poolSeq");
+    S->ctrl = make_dev(sizeof(ControlLine) * kNumControl, "serve.control");
+    S->cfgs = make_dev(sizeof(ConfigDesc) * (size_t)slots, "serve.configs");
+    S->results = make_dev(sizeof(ConfigResult) * (size_t)slots, "serve.results");
+    S->slot_state = make_dev(8 * (size_t)slots, "serve.slotState");
+    S->req_index = make_dev(8 * (size_t)slots, "serve.requestIndex");
+    S->ring_slot = make_dev(4 * (size_t)S->ring, "serve.ringSlot");
+    uint64_t buckets = 0;
+    uint32_t width = 0;
+    if (dedup > 0) {
+        width = (uint32_t)std::min<int64_t>(kDedupBucket, dedup);
+        buckets = (uint64_t)dedup / width;
+        S->dd_tag = make_dev((size_t)buckets * kDedupBucket * 8, "serve.dedupTable");
+        S->dd_key = make_dev((size_t)buckets * kDedupBucket * 64, "serve.dedupTable");
+        HIP_CHECK(hipMemset(S->dd_tag.get(), 0, (size_t)buckets * kDedupBucket * 8));
+    }
+    HIP_CHECK(hipMemset(S->slot_state.get(), 0, 8 * (size_t)slots));
+    HIP_CHECK(hipMemset(S->req_index.get(), 0xFF, 8 * (size_t)slots));
+    S->host_req = make_pinned_coherent(sizeof(ServeRequest) * S->ring);
+    S->host_reply = make_pinned_coherent(sizeof(ServeReply) * (size_t)slots);
+    S->host_ctl = make_pinned_coherent(8 * kServeHostWords);
+    S->h_req = as<ServeRequest>(S->host_req);
+    S->h_reply = as<ServeReply>(S->host_reply);
+    S->h_ctl = as<unsigned long long>(S->host_ctl);
+
+    SearchParams &P = S->P;
+    P.cfgs = as<ConfigDesc>(S->cfgs);
+    P.results = as<ConfigResult>(S->results);
+    P.num_configs = slots;
+    P.num_waves = waves;
+    P.stacks = as<uint32_t>(S->queue_mem);
+    P.stack_cap = stack_cap;
+    P.node_words = nw;
+    P.spi_words = (uint32_t)spi_words_for(max_n);
+    P.inbox_nodes = P.stacks + (size_t)waves * stack_cap * nw;
+    P.inbox = as<unsigned long long>(S->share_lines);
+    P.idle_bits = P.inbox + (size_t)waves * 16;
+    P.idle_words = idle_words;
+    P.pool_nodes = P.stacks + (size_t)waves * (stack_cap + share_max) * nw;
+    P.pool_seq = pool_cap ? as<unsigned long long>(S->pool_seq) : nullptr;
+    P.pool_cap = pool_cap ? pool_cap : 1;
+    P.ctrl = as<ControlLine>(S->ctrl);
+    P.dd_tag = as<unsigned long long>(S->dd_tag);
+    P.dd_key = as<uint32_t>(S->dd_key);
+    P.dd_buckets = buckets;
+    P.dd_width = width;
+    P.epoch = 1;
+    P.cancel_flag = nullptr;
+    P.donate = 1;
+    P.share_mask = (uint32_t)tune.share_interval - 1;
+    P.batch = tune.batch;
+    P.share_max = share_max;
+    P.idle_backoff = (uint32_t)tune.idle_backoff;
+    P.timeout_ticks = (unsigned long long)lifetime_s * 100000000ULL;
+    if (S->learned) {
+        std::lock_guard<std::mutex> mlk(g_model_mutex);
+        if (!g_model.ready) throw std::invalid_argument("no move-ordering model is set (tdp_mlp_set_weights)");
+        if (g_model.device != S->device) throw std::invalid_argument("the move-ordering model is on another device");
+        P.mlp = model_weights();
+    }
+    P.serve = 1;
+    P.slot_state = as<unsigned long long>(S->slot_state);
+    P.req_index = as<unsigned long long>(S->req_index);
+    P.host_req = device_view<ServeRequest>(S->host_req);
+    P.host_reply = device_view<ServeReply>(S->host_reply);
+    P.host_ctl = device_view<unsigned long long>(S->host_ctl);
+    P.ring_slot = as<uint32_t>(S->ring_slot);
+    P.ring = S->ring;
+
+    S->free_slots.resize(slots);
+    for (int i = 0; i < slots; ++i) S->free_slots[i] = slots - 1 - i;
+    S->slot_request.assign(slots, ~0ULL);
+    S->slot_need.assign(slots, 0);
+    S->slot_replied.assign(slots, 0);
+    S->slot_uses.assign(slots, 0);
+
+    HIP_CHECK(hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking));
+    const size_t reset_n = std::max<size_t>({(size_t)slots, (size_t)waves, (size_t)pool_cap, (size_t)kNumControl});
+    hipLaunchKernelGGL(reset_kernel, dim3((unsigned)((reset_n + 255) / 256)), dim3(256), 0, S->stream, P);
+    HIP_CHECK(hipGetLastError());
+    const int blocks = waves / 4 + 1;
+    if (S->learned)
+        hipLaunchKernelGGL(serve_learned_kernel, dim3(blocks), dim3(256), 0, S->stream, P);
+    else
+        hipLaunchKernelGGL(serve_kernel, dim3(blocks), dim3(256), 0, S->stream, P);
+    const hipError_t le = hipGetLastError();
+    if (le != hipSuccess) {
+        (void)hipStreamDestroy(S->stream);
+        HIP_CHECK(le);
+    }
+    g_serve = S;
+}
+
+void serve_stop() {
+    std::shared_ptr<ServeSession> S;
+    {
+        std::lock_guard<std::mutex> lk(g_serve_mutex);
+        S = g_serve;
+        g_serve.reset();
+    }
+    if (!S) return;
+    HIP_CHECK(hipSetDevice(S->device));
+    store_release(&S->h_ctl[kServeHostStop], 1ULL);
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        S->cv.notify_all();
+    }
+    HIP_CHECK(hipStreamSynchronize(S->stream));
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        (void)serve_ended(*S);
+        S->ended = true;
+        S->cv.notify_all();
+    }
+    (void)hipStreamDestroy(S->stream);
+    S->stream = nullptr;
+}
+
+void serve_cancel(ServeSession &S, int64_t ticket);
+int serve_try_admit(ServeSession &S, const ConfigDesc &d, int64_t need, uint64_t *out_request);
+
+// One request through the session; blocks until its reply.  *cancel_ticket
+// (if given) receives request << 16 | slot for tdp_serve_cancel while the
+// request runs.
+void serve_search(ServeSession &S, const CaseInput &c, uint32_t flags, int32_t *out_moves, int32_t *out_depth,
+                  tdp_counters *out_counters, std::atomic<int64_t> *cancel_ticket) {
+    if (c.n > S.max_n) throw std::invalid_argument("performGpuSortingSearch: n exceeds the serving session's max_n");
+    (void)flags;
+    ConfigDesc d;
+    fill_config(d, c);
+    const int64_t need = (int64_t)triple_count(c.n) * (int64_t)S.node_bytes;
+    uint64_t r;
+    int slot;
+    {
+        std::unique_lock<std::mutex> lk(S.mu);
+        while (true) {
+            if (serve_ended(S)) serve_fail(S);
+            serve_reclaim(S);
+            slot = serve_try_admit(S, d, need, &r);
+            if (slot >= 0) break;
+            S.cv.wait_for(lk, std::chrono::microseconds(200));
+        }
+    }
+    if (cancel_ticket) {
+        // a cancel that arrived before the request was published
+        int64_t expect = -1;
+        if (!cancel_ticket->compare_exchange_strong(expect, (int64_t)((r << 16) | (uint64_t)slot)))
+            serve_cancel(S, (int64_t)((r << 16) | (uint64_t)slot));
+    }
+    const auto t0 = std::chrono::steady_clock::now();
+    bool ok = false;
+    while (true) {
+        if (load_acquire(&S.h_reply[slot].seq) == r + 1) {
+            ok = true;
+            break;
+        }
+        if (cancel_ticket && cancel_ticket->load() == -2) break;   // cancelled from another thread
+        const auto us = std::chrono::duration_cast<std::chrono::microseconds>(
+                            std::chrono::steady_clock::now() - t0).count();
+        if (us > 2000) {
+            std::lock_guard<std::mutex> lk(S.mu);
+            if (serve_ended(S)) break;
+        }
+        std::this_thread::sleep_for(std::chrono::microseconds(us < 200 ? 5 : us < 20000 ? 50 : 500));
+    }
+    const bool cancelled = cancel_ticket && cancel_ticket->exchange(-1) == -2;
+    ServeReply rep{};
+    if (ok) std::memcpy(&rep, &S.h_reply[slot], sizeof(rep));
+    {
+        std::lock_guard<std::mutex> lk(S.mu);
+        S.slot_replied[slot] = 1;
+        S.cv.notify_all();
+        if (!ok && S.ended) serve_fail(S);
+    }
+    if (!ok || cancelled || rep.found == 2) throw Cancelled("GPU sorting search cancelled");
+    *out_depth = 0;
+    if (out_counters) {
+        out_counters->expansions = rep.counters[kCntExpansions];
+        out_counters->generated = rep.counters[kCntGenerated];
+        out_counters->admitted = rep.counters[kCntAdmitted];
+        out_counters->dominated = rep.counters[kCntDominated];
+        out_counters->improved = rep.counters[kCntImproved];
+        out_counters->inserted = rep.counters[kCntInserted];
+        out_counters->uninserted = rep.counters[kCntUninserted];
+        out_counters->roots_live = rep.counters[kCntRootsLive];
+    }
+    if (rep.found != 1 || rep.depth <= 0) return;
+    for (int m = 0; m < rep.depth; ++m) triples().unpack(c.n, rep.moves[m], out_moves + 3 * m);
+    *out_depth = rep.depth;
+}
+
+// Admission test and publication of one request (caller holds S.mu):
+// returns its slot, or -1 when it cannot be admitted now.
+int serve_try_admit(ServeSession &S, const ConfigDesc &d, int64_t need, uint64_t *out_request) {
+    const bool ring_free = S.next_request < S.ring ||
+                           load_acquire(&S.h_ctl[kServeHostConsumed]) >= S.next_request - S.ring + 1;
+    const bool room = S.reserved == 0 || S.reserved + need <= S.reserve_limit;
+    if (S.free_slots.empty() || !ring_free || !room) return -1;
+    const int slot = S.free_slots.back();
+    S.free_slots.pop_back();
+    if (++S.slot_uses[slot] >= 0xFFFFu)
+        throw std::runtime_error("serving slot generations exhausted: restart the serving session");
+    const uint64_t r = S.next_request++;
+    S.slot_request[slot] = r;
+    S.slot_need[slot] = need;
+    S.slot_replied[slot] = 0;
+    S.reserved += need;
+    ServeRequest &q = S.h_req[r % S.ring];
+    q.desc = d;
+    q.slot = (uint32_t)slot;
+    store_release<unsigned long long>(&q.seq, r + 1);
+    *out_request = r;
+    return slot;
+}
+
+// A whole batch through the session from one caller: requests are admitted
+// as slots and queue memory free up (continuous batching) while the replies
+// of those in flight are collected.
+void serve_batch(ServeSession &S, const std::vector<CaseInput> &cases, const std::vector<int> &index,
+                 tdp_case_result *out) {
+    const size_t count = cases.size();
+    std::vector<ConfigDesc> descs(count);
+    std::vector<int64_t> need(count);
+    for (size_t c = 0; c < count; ++c) {
+        if (cases[c].n > S.max_n)
+            throw std::invalid_argument("performGpuSortingSearch: n exceeds the serving session's max_n");
+        fill_config(descs[c], cases[c]);
+        need[c] = (int64_t)triple_count(cases[c].n) * (int64_t)S.node_bytes;
+    }
+    struct Flight { size_t c; int slot; uint64_t r; };
+    std::vector<Flight> flight;
+    size_t next = 0, done = 0;
+    while (done < count) {
+        bool progress = false;
+        {
+            std::lock_guard<std::mutex> lk(S.mu);
+            if (serve_ended(S)) serve_fail(S);
+            serve_reclaim(S);
+            while (next < count) {
+                uint64_t r = 0;
+                const int slot = serve_try_admit(S, descs[next], need[next], &r);
+                if (slot < 0) break;
+                flight.push_back({next, slot, r});
+                ++next;
+                progress = true;
+            }
+        }
+        for (size_t q = 0; q < flight.size();) {
+            const Flight f = flight[q];
+            if (load_acquire(&S.h_reply[f.slot].seq) != f.r + 1) {
+                ++q;
+                continue;
+            }
+            ServeReply rep;
+            std::memcpy(&rep, &S.h_reply[f.slot], sizeof(rep));
+            {
+                std::lock_guard<std::mutex> lk(S.mu);
+                S.slot_replied[f.slot] = 1;
+            }
+            tdp_case_result &o = out[index[f.c]];
+            o.found = rep.found == 1 ? 1 : 0;
+            o.depth = rep.found == 1 ? rep.depth : 0;
+            for (int m = 0; m < o.depth; ++m) triples().unpack(cases[f.c].n, rep.moves[m], &o.moves[m][0]);
+            o.counters.expansions = rep.counters[kCntExpansions];
+            o.counters.generated = rep.counters[kCntGenerated];
+            o.counters.admitted = rep.counters[kCntAdmitted];
+            o.counters.dominated = rep.counters[kCntDominated];
+            o.counters.improved = rep.counters[kCntImproved];
+            o.counters.inserted = rep.counters[kCntInserted];
+            o.counters.uninserted = rep.counters[kCntUninserted];
+            o.counters.roots_live = rep.counters[kCntRootsLive];
+            if (rep.found == 2) throw Cancelled("GPU sorting search cancelled");
+            flight[q] = flight.back();
+            flight.pop_back();
+            ++done;
+            progress = true;
+        }
+        if (!progress) std::this_thread::sleep_for(std::chrono::microseconds(20));
+    }
+    std::lock_guard<std::mutex> lk(S.mu);
+    serve_reclaim(S);
+    S.cv.notify_all();
+}
+
+// Publishes a cancellation of request ticket (request << 16 | slot) to the
+// feeder (a ring of kServeCancelEntries entries, consumed within a feeder
+// poll).
+void serve_cancel(ServeSession &S, int64_t ticket) {
+    std::lock_guard<std::mutex> lk(S.mu);
+    const uint64_t k = S.cancels++;
+    store_release(&S.h_ctl[kServeHostCancelRing + (k % kServeCancelEntries)], (unsigned long long)ticket);
+    store_release(&S.h_ctl[kServeHostCancels], (unsigned long long)S.cancels);
+}
+
+
 int search_impl(int slot, const uint8_t *pi, const uint8_t *spi, int n, int e0, float min_rate, int D,
                 int full, uint32_t flags, int32_t *out_moves, int32_t *out_depth, tdp_counters *out_counters) {
     return guarded([&] {
@@ -931,6 +1400,16 @@ int search_impl(int slot, const uint8_t *pi, const uint8_t *spi, int n, int e0, 
         Slot &s = get_slot(slot);
         std::lock_guard<std::mutex> run(s.run_mutex);
         HIP_CHECK(hipSetDevice(s.device));
+        if (auto S = current_serve(); S && S->device == s.device) {
+            // the serving session takes the request (concurrent calls share it)
+            *out_depth = 0;
+            if (out_counters) std::memset(out_counters, 0, sizeof(*out_counters));
+            if (triple_count(n) == 0) return;
+            validate_case(c);
+            s.serve_ticket.store(-1);
+            serve_search(*S, c, flags, out_moves, out_depth, out_counters, &s.serve_ticket);
+            return;
+        }
         clear_cancel(s);
         *out_depth = 0;
         if (out_counters) std::memset(out_counters, 0, sizeof(*out_counters));
@@ -945,6 +1424,7 @@ int search_impl(int slot, const uint8_t *pi, const uint8_t *spi, int n, int e0, 
         *out_depth = r.depth;
     });
 }
+
 
 // ---------------------------------------------------------------- filter
 struct FilterWorkspace {
@@ -1250,7 +1730,77 @@ int tdp_init_slots(int total_slots, int devices_count, int64_t queue_bytes_budge
 }
 
 int tdp_destroy_slots(void) {
-    return guarded([&] { destroy_all_slots(); });
+    return guarded([&] {
+        serve_stop();
+        destroy_all_slots();
+    });
+}
+
+int tdp_serve_start(int64_t queue_bytes, int dedup_table_size, int slots, int max_n, uint32_t flags,
+                    int lifetime_s) {
+    return guarded([&] { serve_start(queue_bytes, dedup_table_size, slots, max_n, flags, lifetime_s); });
+}
+
+int tdp_serve_stop(void) {
+    return guarded([&] { serve_stop(); });
+}
+
+int tdp_serve_search(const uint8_t *pi, const uint8_t *spi, int n, int initial_even_cycles, float min_rate,
+                     int max_moves, int full_sorting, int32_t *out_moves, int32_t *out_depth,
+                     tdp_counters *out_counters) {
+    return guarded([&] {
+        if (!pi || !spi || !out_moves || !out_depth) throw std::invalid_argument("null argument");
+        if (n > kMaxN) throw std::invalid_argument("performGpuSortingSearch: n exceeds MAX_N");
+        if (max_moves <= 0 || max_moves > kMaxDepth)
+            throw std::invalid_argument("performGpuSortingSearch: invalid maximumDepth");
+        auto S = current_serve();
+        if (!S) throw std::invalid_argument("no serving session is running (tdp_serve_start)");
+        HIP_CHECK(hipSetDevice(S->device));
+        CaseInput c{pi, spi, n, initial_even_cycles, max_moves, full_sorting, min_rate};
+        *out_depth = 0;
+        if (out_counters) std::memset(out_counters, 0, sizeof(*out_counters));
+        if (triple_count(n) == 0) return;
+        validate_case(c);
+        serve_search(*S, c, 0u, out_moves, out_depth, out_counters, nullptr);
+    });
+}
+
+int tdp_serve_batch(int count, const uint8_t *pis, const uint8_t *spis, const int32_t *ns, const int32_t *e0s,
+                    const float *min_rates, const int32_t *max_moves, const int32_t *full_sorting,
+                    tdp_case_result *out) {
+    return guarded([&] {
+        if (count < 0) throw std::invalid_argument("negative batch size");
+        if (count == 0) return;
+        if (!pis || !spis || !ns || !e0s || !min_rates || !max_moves || !full_sorting || !out)
+            throw std::invalid_argument("null argument");
+        auto S = current_serve();
+        if (!S) throw std::invalid_argument("no serving session is running (tdp_serve_start)");
+        HIP_CHECK(hipSetDevice(S->device));
+        std::memset(out, 0, sizeof(tdp_case_result) * (size_t)count);
+        std::vector<CaseInput> cases;
+        std::vector<int> index;
+        for (int c = 0; c < count; ++c) {
+            CaseInput ci{pis + 64 * (size_t)c, spis + 64 * (size_t)c, ns[c], e0s[c], max_moves[c], full_sorting[c],
+                         min_rates[c]};
+            validate_case(ci);
+            if (triple_count(ci.n) == 0) continue;
+            cases.push_back(ci);
+            index.push_back(c);
+        }
+        serve_batch(*S, cases, index, out);
+    });
+}
+
+int tdp_serve_stats(uint64_t *completed, uint64_t *in_flight, int64_t *reserved_bytes) {
+    return guarded([&] {
+        auto S = current_serve();
+        if (!S) throw std::invalid_argument("no serving session is running (tdp_serve_start)");
+        std::lock_guard<std::mutex> lk(S->mu);
+        serve_reclaim(*S);
+        if (completed) *completed = S->completed;
+        if (in_flight) *in_flight = (uint64_t)(S->slots - (int)S->free_slots.size());
+        if (reserved_bytes) *reserved_bytes = S->reserved;
+    });
 }
 
 int tdp_search(int slot, const uint8_t *pi, const uint8_t *spi, int n, int initial_even_cycles, float min_rate,
@@ -1329,6 +1879,13 @@ void tdp_cancel(int slot) {
     // into the device flag on the slot's cancel stream.  Never throws.
     try {
         Slot &s = get_slot(slot);
+        if (auto S = current_serve(); S && S->device == s.device) {
+            // serving: the running request is cancelled through the feeder;
+            // one not yet published is cancelled when it is
+            const int64_t t = s.serve_ticket.exchange(-2);
+            if (t >= 0) serve_cancel(*S, t);
+            return;
+        }
         s.cancelled.store(true, std::memory_order_release);
         int cur = -1;
         (void)hipGetDevice(&cur);

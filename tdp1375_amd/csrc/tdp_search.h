@@ -54,10 +54,28 @@ struct SearchParams {
     uint32_t idle_backoff;        // longest idle sleep, in units of s_sleep 127
     unsigned long long timeout_ticks;  // watchdog: 100 MHz ticks per wave, 0 = none
     MlpWeights mlp;               // learned ordering (search_learned_kernel only)
+
+    // device-resident serving (serve kernels only): num_configs device slots
+    // (cfgs, results), their state words, the request index each holds, the
+    // pinned host request ring (ring entries), reply slots and control words
+    int serve;
+    unsigned long long *slot_state;     // num_configs
+    unsigned long long *req_index;      // num_configs
+    const ServeRequest *host_req;       // ring entries
+    ServeReply *host_reply;             // num_configs
+    unsigned long long *host_ctl;       // kServeHostWords
+    uint32_t *ring_slot;                // ring entries: device slot of each published request
+    uint32_t ring;
 };
 
 __global__ void search_kernel(SearchParams P);
 __global__ void search_learned_kernel(SearchParams P);
+// Device-resident serving: the search waves (num_waves, in blocks of 4) plus
+// one more block whose first wave feeds requests from the host ring.  Runs
+// until the host sets kServeHostStop, the queues overflow or the watchdog
+// fires.
+__global__ void serve_kernel(SearchParams P);
+__global__ void serve_learned_kernel(SearchParams P);
 __global__ void reset_kernel(SearchParams P);
 __global__ void eval_children_kernel(const uint8_t *pi, const uint8_t *spi, int n,
                                      int T, int e, int32_t *out, int e0, int d, int D, uint32_t *features);

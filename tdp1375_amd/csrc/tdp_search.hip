@@ -60,7 +60,9 @@ struct WaveLds {
     uint32_t ckey[16][kWave];                // child spi words of each lane's staged child
     unsigned long long cnt[kNumCounters];    // counters of the current configuration
     unsigned long long pend[kNumCounters];   // counters of the current expansion
-    unsigned long long live;                 // pushes - pops of the current configuration
+    unsigned long long live;                 // pushes - pops of the current configuration (serving:
+                                             // pushes not yet added to the request's count)
+    unsigned long long takes;                // serving: nodes taken, not yet subtracted
     unsigned long long traffic[6];           // pops, pushes, tag reads, key compares, key writes,
                                              // evictions
     unsigned long long phase[kNumPhases];    // 10 ns ticks per phase (TDP_PHASE_TIMING builds)
@@ -147,8 +149,29 @@ __device__ __forceinline__ bool cas64(unsigned long long *p, unsigned long long 
                                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Pinned host memory of a serving session (request ring, replies, control
+// words) is accessed with system-scope atomics, which bypass the GPU caches.
+__device__ __forceinline__ unsigned long long ld_sys(const unsigned long long *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+__device__ __forceinline__ uint32_t ld_sys32(const uint32_t *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+__device__ __forceinline__ void st_sys(unsigned long long *p, unsigned long long v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+__device__ __forceinline__ void st_sys32(uint32_t *p, uint32_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 __device__ __forceinline__ unsigned long long ctl_load(const SearchParams &P, int idx) {
     return ld_relaxed(&P.ctrl[idx].v);
+}
+// A control word read by lane 0 and broadcast (wave-uniform).
+__device__ __forceinline__ unsigned long long ctl_load_uniform(const SearchParams &P, int idx) {
+    unsigned long long v = 0;
+    if ((threadIdx.x & 63) == 0) v = ctl_load(P, idx);
+    return ((unsigned long long)(uint32_t)__shfl((int)(v >> 32), 0) << 32) | (uint32_t)__shfl((int)v, 0);
 }
 
 __device__ __forceinline__ uint64_t fmix64(uint64_t h) {
@@ -357,6 +380,7 @@ struct WaveState {
     const uint32_t *colex3;        // workgroup's colex triple table (LDS)
     uint32_t lo, hi;               // private stack window [lo, hi), wraps mod 2^32
     int cur_cfg;
+    uint32_t cur_gen;              // serving: generation of the slot cur_cfg
     bool configs_left;
     bool stop;
     unsigned long long deadline;   // s_memrealtime ticks (100 MHz); 0 = none
@@ -371,15 +395,37 @@ __device__ __forceinline__ void set_watchdog(const SearchParams &P, WaveState &W
     W.stop = true;
 }
 
+template <bool kServe>
 __device__ __forceinline__ void flush_counters(const SearchParams &P, WaveState &W, WaveLds &L) {
     wave_sync();
     if (W.cur_cfg >= 0 && W.lane < kNumCounters) {
         const unsigned long long v = L.cnt[W.lane];
         if (v) fetch_add(&P.results[W.cur_cfg].counters[W.lane], v);
     }
-    if (W.cur_cfg >= 0 && W.lane == kNumCounters && L.live) fetch_add(&P.results[W.cur_cfg].live, L.live);
+    if (!kServe) {
+        if (W.cur_cfg >= 0 && W.lane == kNumCounters && L.live) fetch_add(&P.results[W.cur_cfg].live, L.live);
+        if (W.lane == kNumCounters) L.live = 0;
+    }
     if (W.lane < kNumCounters) L.cnt[W.lane] = 0;
-    if (W.lane == kNumCounters) L.live = 0;
+    wave_sync();
+}
+
+// ---- serving: the outstanding-node count of a request.  results[c].live
+// counts the request's nodes that exist (on a stack, in an inbox or the spill
+// pool, or taken by a wave whose batch has not ended).  A push is added
+// before the node can leave its wave (serve_flush_pushes ahead of every
+// hand-off or spill) and a take is subtracted only at the end of the batch,
+// together with the pushes of its children, so the count never falls below
+// the true number: it reaches zero exactly once, when the last node is gone,
+// and the wave whose subtraction gets it there reports the request drained.
+__device__ __forceinline__ void serve_flush_pushes(const SearchParams &P, const WaveState &W, WaveLds &L) {
+    wave_sync();
+    const unsigned long long add = L.live;
+    wave_sync();
+    if (W.lane == 0 && add) {
+        L.live = 0;
+        fetch_add(&P.results[W.cur_cfg].live, add);
+    }
     wave_sync();
 }
 
@@ -569,9 +615,11 @@ __device__ __forceinline__ bool take_one(const SearchParams &P, WaveState &W, Wa
 // Ensures room for one push on the private stack: a full stack hands its
 // oldest nodes to an idle wave, else spills them to the pool, else the launch
 // stops with a queue overflow.
-__device__ __forceinline__ bool make_room(const SearchParams &P, WaveState &W) {
+template <bool kServe>
+__device__ __forceinline__ bool make_room(const SearchParams &P, WaveState &W, WaveLds &L) {
     if (W.hi - W.lo < P.stack_cap) return true;
     unsigned long long reason = 1;   // 1: no sharing, 2: pool full
+    if (kServe) serve_flush_pushes(P, W, L);   // before any node leaves the wave
     if (P.donate) {
         uint32_t most = (W.hi - W.lo) / 2;
         if (most > P.share_max) most = P.share_max;
@@ -624,9 +672,10 @@ __device__ __forceinline__ uint32_t child_word(const uint32_t *pw, int lane, int
 }
 
 // Pushes the child (t, i, j, k, even) of node pw.
+template <bool kServe>
 __device__ __forceinline__ void push_child(const SearchParams &P, WaveState &W, WaveLds &L, const uint32_t *pw,
                                            int n, int sw, int t, int i, int j, int k, int even, const ChildMove &m) {
-    if (!make_room(P, W)) return;
+    if (!make_room<kServe>(P, W, L)) return;
     const uint32_t w = child_word(pw, W.lane, n, sw, t, i, j, k, even, m);
     uint32_t *dst = stack_slot(P, W, W.hi);
     if (W.lane < (int)P.node_words) dst[W.lane] = w;
@@ -637,8 +686,9 @@ __device__ __forceinline__ void push_child(const SearchParams &P, WaveState &W, 
 
 
 // Pushes a copy of node nd (a node of a batch that was not expanded).
+template <bool kServe>
 __device__ __forceinline__ void push_node(const SearchParams &P, WaveState &W, WaveLds &L, const uint32_t *nd) {
-    if (!make_room(P, W)) return;
+    if (!make_room<kServe>(P, W, L)) return;
     uint32_t *dst = stack_slot(P, W, W.hi);
     if (W.lane < (int)P.node_words) dst[W.lane] = nd[W.lane];
     W.hi++;
@@ -688,8 +738,13 @@ __device__ __forceinline__ unsigned long long key_pair(const WaveLds &L, int lan
     return lo | (hi << 32);
 }
 
+// Serving (P.serve): the epoch field of an identity is the generation of the
+// request's slot (gen), a free entry is a zero tag, and an entry is evictable
+// first when its slot has moved to another request (other generation) or its
+// request is done (found, cancelled or drained).
+template <bool kServe>
 __device__ __forceinline__ int dedup_admit(const SearchParams &P, WaveLds &L, int lane, bool active, uint32_t cfg,
-                                           int nw, int depth) {
+                                           int nw, int depth, uint32_t gen) {
     if (P.dd_buckets == 0) return active ? kDedupUninserted : kDedupDominated;
     // two 32-bit multiplicative lanes over the even and the odd key words
     // (each step a bijection of the lane state, so keys that differ in one
@@ -703,12 +758,12 @@ __device__ __forceinline__ int dedup_admit(const SearchParams &P, WaveLds &L, in
         h2 ^= h2 >> 13;
     }
     const uint64_t h = fmix64(((uint64_t)h1 << 32) | h2);
-    const unsigned long long ident = dedup_ident(P.epoch, cfg, h);
+    const unsigned long long ident = dedup_ident(gen, cfg, h);
     const unsigned long long bucket = __umul64hi(h, P.dd_buckets);
     unsigned long long *tags = P.dd_tag + bucket * kDedupBucket;
     const uint32_t *keys = P.dd_key + bucket * kDedupBucket * 16;
     const int width = (int)P.dd_width;
-    const unsigned long long epoch = P.epoch;
+    const unsigned long long epoch = gen;
     int res = active ? kDedupPending : kDedupDominated;
     int rounds = 0;
     while (true) {
@@ -719,6 +774,7 @@ __device__ __forceinline__ int dedup_admit(const SearchParams &P, WaveLds &L, in
             unsigned long long t[kDedupBucket];
 #pragma unroll
             for (int q = 0; q < kDedupBucket; ++q) t[q] = q < width ? ld_relaxed(tags + q) : 0ULL;
+
             bool retry = false, busy = false;
             uint32_t cand = 0;   // ready entries with this identity
 #pragma unroll
@@ -775,7 +831,7 @@ __device__ __forceinline__ int dedup_admit(const SearchParams &P, WaveLds &L, in
                     int victim = -1;
 #pragma unroll
                     for (int q = kDedupBucket - 1; q >= 0; --q)
-                        if (q < width && (t[q] >> 48) != epoch) victim = q;
+                        if (q < width && (kServe ? t[q] == 0ULL : (t[q] >> 48) != epoch)) victim = q;
                     bool live = false;
                     if (victim < 0) {
                         // an entry of a configuration already found
@@ -783,9 +839,15 @@ __device__ __forceinline__ int dedup_admit(const SearchParams &P, WaveLds &L, in
 #pragma unroll
                         for (int q = 0; q < kDedupBucket; ++q) {
                             if (q < width && (t[q] & kTagReady) && !((same_ident >> q) & 1u)) {
-                                const uint32_t f = (uint32_t)__hip_atomic_load(
-                                    &P.results[(t[q] >> 32) & 0xFFFFu].found, __ATOMIC_RELAXED,
-                                    __HIP_MEMORY_SCOPE_AGENT);
+                                const unsigned long long c = (t[q] >> 32) & 0xFFFFu;
+                                uint32_t f;
+                                if (kServe) {
+                                    const unsigned long long st = ld_relaxed(&P.slot_state[c]);
+                                    f = ((st >> 2) != (t[q] >> 48) || (st & kSlotDone)) ? 1u : 0u;
+                                } else {
+                                    f = (uint32_t)__hip_atomic_load(&P.results[c].found, __ATOMIC_RELAXED,
+                                                                    __HIP_MEMORY_SCOPE_AGENT);
+                                }
                                 found_mask |= f ? (1u << q) : 0u;
                             }
                         }
@@ -831,9 +893,9 @@ __device__ __forceinline__ int dedup_admit(const SearchParams &P, WaveLds &L, in
 
 // Deduplicates and pushes the staged children (descending triple order).
 // root_stage: children of the start node (not counted as admitted).
-template <bool kLearned>
-__device__ __forceinline__ void flush_stage(const SearchParams &P, WaveState &W, WaveLds &L, int count, int n, int sw,
-                            int T, bool root_stage) {
+template <bool kLearned, bool kServe>
+__device__ __forceinline__ void flush_stage(const SearchParams &P, WaveState &W, WaveLds &L, const ConfigDesc &C,
+                                            int count, int n, int sw, int T, bool root_stage) {
     const uint32_t cfg = L.node[0][0];   // every node of a batch has the same configuration
     for (int base = 0; base < count && !W.stop; base += kWave) {
         const int idx = base + W.lane;
@@ -853,7 +915,7 @@ __device__ __forceinline__ void flush_stage(const SearchParams &P, WaveState &W,
         const uint32_t mv2 = m.na | (m.nb << 8) | (m.nc << 16);
         PHASE_T0();
         for (int q = 0; q < sw; ++q) L.ckey[q][W.lane] = child_spi_word(nd + kNodeHeaderWords, q, m);
-        const int r = dedup_admit(P, L, W.lane, act, cfg, sw, d + 1);
+        const int r = dedup_admit<kServe>(P, L, W.lane, act, cfg, sw, d + 1, kServe ? W.cur_gen : P.epoch);
         PHASE_ADD(L, kPhaseDedup);
         const bool admit = act && r != kDedupDominated;
         if (!root_stage) {
@@ -874,7 +936,6 @@ __device__ __forceinline__ void flush_stage(const SearchParams &P, WaveState &W,
             // least two (MFMA MLP over tiles of 16 of them, compacted in lane
             // order)
             PHASE_T0();
-            const ConfigDesc &C = P.cfgs[cfg];
             const int e = (int)((nd[1] >> 8) & 0xFF);
             uint32_t f[kFeatWords];
             const uint32_t *pinfo = L.posinfo[st >> 24];   // the child's parent
@@ -970,7 +1031,7 @@ __device__ __forceinline__ void flush_stage(const SearchParams &P, WaveState &W,
                     lds_add(&L.traffic[1], 2);
                     lds_add(&L.live, 2ULL);
                 } else {
-                    push_child(P, W, L, L.node[cst >> 24], n, sw, ct, (int)(cst & 63u), (int)((cst >> 6) & 63u),
+                    push_child<kServe>(P, W, L, L.node[cst >> 24], n, sw, ct, (int)(cst & 63u), (int)((cst >> 6) & 63u),
                                (int)((cst >> 12) & 63u), (int)((cst >> 18) & 63u), cm);
                 }
                 if (W.stop) break;
@@ -980,6 +1041,7 @@ __device__ __forceinline__ void flush_stage(const SearchParams &P, WaveState &W,
     }
 }
 
+template <bool kServe>
 __device__ __forceinline__ void report_found(const SearchParams &P, WaveState &W, const uint32_t *nd, int cfg,
                                              int depth, int extra_t) {
     int won = 0;
@@ -995,13 +1057,73 @@ __device__ __forceinline__ void report_found(const SearchParams &P, WaveState &W
     won = __shfl(won, 0);
     if (won) {
         const int pd = (int)(nd[1] & 0xFF);
+        uint32_t mv = 0;
         if (W.lane < pd) {
             const uint32_t w = nd[2 + (W.lane >> 1)];
-            P.results[cfg].moves[W.lane] = (W.lane & 1) ? (w >> 16) : (w & 0xFFFF);
+            mv = (W.lane & 1) ? (w >> 16) : (w & 0xFFFF);
         } else if (W.lane == pd && extra_t >= 0) {
-            P.results[cfg].moves[W.lane] = (uint32_t)extra_t;
+            mv = (uint32_t)extra_t;
+        }
+        if (W.lane < depth) P.results[cfg].moves[W.lane] = mv;
+        if (kServe) {
+            // the verdict goes to the host now; the slot's dedup entries may go
+            ServeReply *hr = P.host_reply + cfg;
+            if (W.lane < depth) st_sys32(&hr->moves[W.lane], mv);
+            // counters of the batches already ended (later ones are not in the reply)
+            if (W.lane < kNumCounters) st_sys(&hr->counters[W.lane], ld_relaxed(&P.results[cfg].counters[W.lane]));
+            if (W.lane == 0) {
+                st_sys32(reinterpret_cast<uint32_t *>(&hr->found), 1u);
+                st_sys32(reinterpret_cast<uint32_t *>(&hr->depth), (uint32_t)depth);
+                __hip_atomic_fetch_or(&P.slot_state[cfg], kSlotDone, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            complete_before();
+            if (W.lane == 0) st_sys(&hr->seq, ld_relaxed(&P.req_index[cfg]) + 1);
         }
     }
+}
+
+// Serving: every node of the request in slot cfg is gone.  A request not
+// found (or cancelled) gets its reply now, with its final counters; then the
+// slot is marked done and handed back to the host.
+__device__ __forceinline__ void serve_drained(const SearchParams &P, WaveState &W, int cfg) {
+    ServeReply *hr = P.host_reply + cfg;
+    complete_after();
+    const int found = __hip_atomic_load(&P.results[cfg].found, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned long long r = ld_relaxed(&P.req_index[cfg]);
+    if (found != 1) {
+        if (W.lane < kNumCounters) st_sys(&hr->counters[W.lane], ld_relaxed(&P.results[cfg].counters[W.lane]));
+        if (W.lane == 0) {
+            st_sys32(reinterpret_cast<uint32_t *>(&hr->found), (uint32_t)found);
+            st_sys32(reinterpret_cast<uint32_t *>(&hr->depth), 0u);
+        }
+        complete_before();
+        if (W.lane == 0) st_sys(&hr->seq, r + 1);
+    }
+    if (W.lane == 0)
+        __hip_atomic_fetch_or(&P.slot_state[cfg], kSlotDone, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    complete_before();
+    if (W.lane == 0) st_sys(&hr->drained, r + 1);
+}
+
+// Serving: end of a batch (or root expansion) of the current request: its
+// counters, then its pushes minus its takes; the subtraction that brings the
+// outstanding count to zero reports the request drained.
+__device__ __forceinline__ void serve_flush(const SearchParams &P, WaveState &W, WaveLds &L) {
+    flush_counters<true>(P, W, L);
+    const long long delta = (long long)L.live - (long long)L.takes;
+    wave_sync();
+    int drained = 0;
+    if (W.lane == 0) {
+        L.live = 0;
+        L.takes = 0;
+        complete_before();   // the counters land before the count can reach zero
+        if (delta != 0) {
+            const unsigned long long old = fetch_add(&P.results[W.cur_cfg].live, (unsigned long long)delta);
+            drained = (old + (unsigned long long)delta == 0ULL) ? 1 : 0;
+        }
+    }
+    wave_sync();
+    if (__shfl(drained, 0)) serve_drained(P, W, W.cur_cfg);
 }
 
 // Children of an expansion that needs a gain of +2.  Such a child either takes
@@ -1154,7 +1276,7 @@ __device__ __forceinline__ int cycle_local_children(WaveState &W, WaveLds &L, co
 // stages the admissible ones behind the scount entries already staged,
 // flushing the stage only when it would overflow.  Normal mode: returns the
 // goal triple (terminal expansion) or -1.  Root mode: stages every live root.
-template <bool kLearned>
+template <bool kLearned, bool kServe>
 __device__ __forceinline__ int expand_stage(const SearchParams &P, WaveState &W, WaveLds &L, const ConfigDesc &C,
                                             const uint32_t *nd, int b, bool root_mode, int &scount) {
     const int n = (int)C.n;
@@ -1236,7 +1358,7 @@ __device__ __forceinline__ int expand_stage(const SearchParams &P, WaveState &W,
         if (local == kLocalFull) {
             // the children of earlier nodes of the batch leave no room
             wave_sync();
-            flush_stage<kLearned>(P, W, L, scount, n, sw, T, root_mode);
+            flush_stage<kLearned, kServe>(P, W, L, C, scount, n, sw, T, root_mode);
             scount = 0;
             if (W.stop) return -1;
             local = cycle_local_children(W, L, nd, b, pb, sinfo, n, sw, T, e, gmask, amask, root_mode, scount, goal_t);
@@ -1246,7 +1368,7 @@ __device__ __forceinline__ int expand_stage(const SearchParams &P, WaveState &W,
         if (scount > kStage - kWave) {
             // a chunk of the scan may stage up to kWave children
             wave_sync();
-            flush_stage<kLearned>(P, W, L, scount, n, sw, T, root_mode);
+            flush_stage<kLearned, kServe>(P, W, L, C, scount, n, sw, T, root_mode);
             scount = 0;
             if (W.stop) return -1;
         }
@@ -1254,7 +1376,7 @@ __device__ __forceinline__ int expand_stage(const SearchParams &P, WaveState &W,
         while (scan_children(W.colex, n, T, W.lane, mb, qb, visit) == kScanPause) {
             // stage full: flush, then resume the scan at the next chunk
             wave_sync();
-            flush_stage<kLearned>(P, W, L, scount, n, sw, T, root_mode);
+            flush_stage<kLearned, kServe>(P, W, L, C, scount, n, sw, T, root_mode);
             scount = 0;
             if (W.stop) return -1;
         }
@@ -1266,7 +1388,7 @@ __device__ __forceinline__ int expand_stage(const SearchParams &P, WaveState &W,
 
 __device__ __forceinline__ void switch_config(const SearchParams &P, WaveState &W, WaveLds &L, int cfg) {
     if (cfg != W.cur_cfg) {
-        flush_counters(P, W, L);
+        flush_counters<false>(P, W, L);
         W.cur_cfg = cfg;
     }
 }
@@ -1275,40 +1397,186 @@ __device__ __forceinline__ bool config_found(const SearchParams &P, int cfg) {
     return __hip_atomic_load(&P.results[cfg].found, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
 }
 
+template <bool kServe>
 __device__ __forceinline__ bool should_stop(const SearchParams &P) {
     int s = 0;
     if ((threadIdx.x & 63) == 0) {
         s = ctl_load(P, kCtlStop) != 0;
         if (P.cancel_flag)
             s |= __hip_atomic_load(P.cancel_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
-        s |= ctl_load(P, kCtlResolved) >= (unsigned long long)P.num_configs;
+        // a serving session has no fixed set of configurations to resolve
+        if (!kServe) s |= ctl_load(P, kCtlResolved) >= (unsigned long long)P.num_configs;
     }
     return __shfl(s, 0) != 0;
 }
 
+// Serving: the slot's descriptor into the wave's LDS copy, read with
+// device-coherent loads (the feeder rewrites a slot's descriptor for each
+// new request, so no cached copy of an earlier one may be used).
+__device__ __forceinline__ void serve_load_config(const SearchParams &P, const WaveState &W, ConfigDesc &copy,
+                                                  int cfg) {
+    const uint32_t *src = reinterpret_cast<const uint32_t *>(P.cfgs + cfg);
+    uint32_t *dst = reinterpret_cast<uint32_t *>(&copy);
+    constexpr int kWords = (int)(sizeof(ConfigDesc) / 4);
+    for (int w = W.lane; w < kWords; w += kWave) dst[w] = ld32(src + w);
+    wave_sync();
+}
+
+// Serving: the state word of slot cfg; a wave whose LDS copy belongs to
+// another configuration or generation reloads it.  Returns true when the
+// request is done (its remaining nodes are dropped).
+__device__ __forceinline__ bool serve_switch(const SearchParams &P, WaveState &W, WaveLds &L, ConfigDesc &copy,
+                                             int cfg) {
+    unsigned long long st = 0;
+    if (W.lane == 0) st = ld_relaxed(&P.slot_state[cfg]);
+    const uint32_t lo = (uint32_t)__shfl((int)(uint32_t)st, 0);
+    const uint32_t gen = lo >> 2;
+    if (cfg != W.cur_cfg || gen != W.cur_gen) {
+        flush_counters<true>(P, W, L);
+        W.cur_cfg = cfg;
+        W.cur_gen = gen;
+        serve_load_config(P, W, copy, cfg);
+    }
+    return (lo & (uint32_t)kSlotDone) != 0;
+}
+
+// Serving: claims the next request the feeder has published, if any.
+// Returns its slot or -1 (wave-uniform).
+__device__ __forceinline__ int serve_claim(const SearchParams &P, const WaveState &W) {
+    int slot = -1;
+    if (W.lane == 0) {
+        const unsigned long long pub = ctl_load(P, kCtlPublished);
+        complete_after();
+        const unsigned long long next = ctl_load(P, kCtlNextConfig);
+        if (next < pub && cas64(&P.ctrl[kCtlNextConfig].v, next, next + 1)) {
+            complete_after();
+            slot = (int)ld32(&P.ring_slot[next % P.ring]);
+        }
+    }
+    return uniform(__shfl(slot, 0));
+}
+
+// Serving: the feeder wave.  Copies each request the host publishes in its
+// ring (in ring order) into the device slot the host chose, resets the
+// slot's result and outstanding count, starts a new generation of the slot,
+// then publishes it to the search waves.  Ends on the host's stop word (and
+// then stops the search waves), on any other stop, or at the watchdog.
+__device__ __forceinline__ void serve_feeder(const SearchParams &P, WaveState &W) {
+    unsigned long long next = 0, cancels = 0;
+    int backoff = 1;
+    constexpr int kWords = (int)(sizeof(ConfigDesc) / 4);
+    while (true) {
+        unsigned long long seq = 0, cancel_pub = 0;
+        int host_stop = 0, dev_stop = 0;
+        if (W.lane == 0) {
+            host_stop = ld_sys(&P.host_ctl[kServeHostStop]) != 0;
+            dev_stop = ctl_load(P, kCtlStop) != 0;
+            seq = ld_sys(&P.host_req[next % P.ring].seq);
+            cancel_pub = ld_sys(&P.host_ctl[kServeHostCancels]);
+            // cancellations: entry = request << 16 | slot; a request still in
+            // its slot is marked cancelled (found = 2) and done, so its nodes
+            // are dropped and the wave that drains it replies "cancelled"
+            for (; cancels < cancel_pub; ++cancels) {
+                complete_after();
+                const unsigned long long e =
+                    ld_sys(&P.host_ctl[kServeHostCancelRing + (cancels % kServeCancelEntries)]);
+                const int slot = (int)(e & 0xFFFFULL);
+                if (slot < P.num_configs && ld_relaxed(&P.req_index[slot]) == (e >> 16)) {
+                    int expected = 0;
+                    __hip_atomic_compare_exchange_strong(&P.results[slot].found, &expected, 2, __ATOMIC_RELAXED,
+                                                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    __hip_atomic_fetch_or(&P.slot_state[slot], kSlotDone, __ATOMIC_RELAXED,
+                                          __HIP_MEMORY_SCOPE_AGENT);
+                }
+            }
+        }
+        if (__shfl(host_stop, 0)) {
+            if (W.lane == 0) __hip_atomic_store(&P.ctrl[kCtlStop].v, 3ULL, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            break;
+        }
+        if (__shfl(dev_stop, 0)) break;
+        if (past_deadline(W)) {
+            set_watchdog(P, W);
+            break;
+        }
+        seq = ((unsigned long long)(uint32_t)__shfl((int)(seq >> 32), 0) << 32) | (uint32_t)__shfl((int)seq, 0);
+        // at most one ring of published requests not yet claimed (the
+        // claimers read their slot from ring_slot)
+        const bool ahead = next - ctl_load_uniform(P, kCtlNextConfig) >= (unsigned long long)P.ring;
+        if (seq != next + 1 || ahead) {
+            for (int z = 0; z < backoff; ++z) __builtin_amdgcn_s_sleep(32);
+            if (backoff < 16) backoff *= 2;
+            continue;
+        }
+        backoff = 1;
+        complete_after();
+        const ServeRequest *rq = P.host_req + next % P.ring;
+        const int slot = uniform((int)ld_sys32(&rq->slot));
+        const uint32_t *src = reinterpret_cast<const uint32_t *>(&rq->desc);
+        // the session's descriptor slots are written only by this wave
+        uint32_t *dst = reinterpret_cast<uint32_t *>(const_cast<ConfigDesc *>(P.cfgs + slot));
+        for (int w = W.lane; w < kWords; w += kWave) st32(dst + w, ld_sys32(src + w));
+        ConfigResult &res = P.results[slot];
+        if (W.lane < kNumCounters) st64(&res.counters[W.lane], 0ULL);
+        if (W.lane < 16) st32(&res.moves[W.lane], 0u);
+        if (W.lane == 0) {
+            st32(reinterpret_cast<uint32_t *>(&res.found), 0u);
+            st32(reinterpret_cast<uint32_t *>(&res.depth), 0u);
+            st64(&res.live, 0ULL);
+            st64(&P.req_index[slot], next);
+            st32(&P.ring_slot[next % P.ring], (uint32_t)slot);
+            // next generation of the slot (1..65535): the previous request's
+            // dedup entries no longer match and may be evicted
+            const unsigned long long st = ld_relaxed(&P.slot_state[slot]);
+            unsigned long long gen = ((st >> 2) + 1) & 0xFFFFULL;
+            if (gen == 0) gen = 1;
+            st64(&P.slot_state[slot], gen << 2);
+        }
+        complete_before();
+        if (W.lane == 0) {
+            st64(&P.ctrl[kCtlPublished].v, next + 1);
+            st_sys(&P.host_ctl[kServeHostConsumed], next + 1);
+        }
+        ++next;
+    }
+}
+
 }  // namespace
 
-template <bool kLearned>
+template <bool kLearned, bool kServe>
 __device__ __forceinline__ void search_body(const SearchParams &P) {
     __shared__ WaveLds lds[kWavesPerBlock];
     __shared__ uint16_t colex[kColexPairs];
     __shared__ uint32_t colex3[kCycleLocalMax];
+    // serving: each wave's copy of the descriptor of its current request
+    __shared__ ConfigDesc cfg_copy[kServe ? kWavesPerBlock : 1];
     fill_colex(colex, colex3);
     const int wib = threadIdx.x >> 6;
     WaveLds &L = lds[wib];
     WaveState W;
     W.lane = threadIdx.x & 63;
-    W.wave = blockIdx.x * kWavesPerBlock + wib;
+    // a serving launch: block 0 (dispatched first, so it is resident whatever
+    // else runs) holds the feeder wave; the search waves follow
+    W.wave = (int)(kServe ? blockIdx.x - 1 : blockIdx.x) * kWavesPerBlock + wib;
+    if (kServe && blockIdx.x == 0) {
+        if (wib != 0) return;
+        W.deadline = P.timeout_ticks ? __builtin_amdgcn_s_memrealtime() + P.timeout_ticks : 0ULL;
+        W.stop = false;
+        serve_feeder(P, W);
+        return;
+    }
     if (W.wave >= P.num_waves) return;
     W.colex = colex;
     W.colex3 = colex3;
     W.lo = W.hi = 0;
     W.cur_cfg = -1;
+    W.cur_gen = 0;
     W.configs_left = true;
     W.stop = false;
     W.deadline = P.timeout_ticks ? __builtin_amdgcn_s_memrealtime() + P.timeout_ticks : 0ULL;
     if (W.lane < kNumCounters) L.cnt[W.lane] = 0;
     if (W.lane == 0) L.live = 0;
+    if (W.lane == 0) L.takes = 0;
     if (W.lane < 6) L.traffic[W.lane] = 0;
     if (W.lane < kNumPhases) L.phase[W.lane] = 0;
     wave_sync();
@@ -1319,7 +1587,7 @@ __device__ __forceinline__ void search_body(const SearchParams &P) {
     int backoff = 1;
 
     while (true) {
-        if (W.stop || ((++polls & 7) == 0 && should_stop(P))) break;
+        if (W.stop || ((++polls & 7) == 0 && should_stop<kServe>(P))) break;
         if ((polls & 63) == 0 && past_deadline(W)) {
             set_watchdog(P, W);
             break;
@@ -1351,17 +1619,54 @@ __device__ __forceinline__ void search_body(const SearchParams &P) {
             lds_add(&L.traffic[0], (unsigned long long)batch);
             have = true;
             PHASE_ADD(L, kPhasePop);
-        } else if (W.configs_left) {
+        } else if (kServe ? (ctl_load_uniform(P, kCtlPublished) > ctl_load_uniform(P, kCtlNextConfig))
+                          : W.configs_left) {
             int c = 0;
-            if (W.lane == 0) c = (int)fetch_add(&P.ctrl[kCtlNextConfig].v, 1ULL);
-            c = uniform(__shfl(c, 0));
-            if (c >= P.num_configs) {
-                W.configs_left = false;
-                continue;
+            if constexpr (kServe) {
+                if (idle) {
+                    // leave the idle set first; a wave already claimed by a
+                    // donor keeps its place and takes the nodes on their way
+                    int mine = 0;
+                    if (W.lane == 0) {
+                        const unsigned long long bit = 1ULL << (W.wave & 63);
+                        mine = (__hip_atomic_fetch_and(idle_word(P, W.wave >> 6), ~bit, __ATOMIC_RELAXED,
+                                                       __HIP_MEMORY_SCOPE_AGENT) & bit) ? 1 : 0;
+                        if (mine) fetch_add(&P.ctrl[kCtlIdle].v, ~0ULL);
+                    }
+                    if (!__shfl(mine, 0)) {
+                        // a donor has claimed this wave: its nodes are on the
+                        // way to the inbox (the donor took the wave out of the
+                        // idle count)
+                        while (!receive_nodes(P, W)) {
+                            if (past_deadline(W)) {
+                                set_watchdog(P, W);
+                                break;
+                            }
+                            __builtin_amdgcn_s_sleep(2);
+                        }
+                        idle = false;
+                        continue;
+                    }
+                    idle = false;
+                }
+                c = serve_claim(P, W);
+                if (c < 0) continue;
+                serve_switch(P, W, L, cfg_copy[wib], c);
+                // the start node counts as pushed before its root expansion
+                // takes it (flushed with the roots at the end)
+                if (W.lane == 0) fetch_add(&P.results[c].live, 1ULL);
+                if (W.lane == 0) L.takes = 1;
+            } else {
+                if (W.lane == 0) c = (int)fetch_add(&P.ctrl[kCtlNextConfig].v, 1ULL);
+                c = uniform(__shfl(c, 0));
+                if (c >= P.num_configs) {
+                    W.configs_left = false;
+                    continue;
+                }
+                // start node of configuration c, expanded in root mode
+                switch_config(P, W, L, c);
             }
-            // start node of configuration c, expanded in root mode
-            switch_config(P, W, L, c);
-            const ConfigDesc &C = P.cfgs[c];
+            const ConfigDesc &C = kServe ? cfg_copy[wib] : P.cfgs[c];
             const int sw = (int)P.spi_words;
             if (W.lane < (int)P.node_words) {
                 uint32_t w = 0;
@@ -1377,13 +1682,14 @@ __device__ __forceinline__ void search_body(const SearchParams &P) {
             if (C.triple_count > 0) {
                 PHASE_T0();
                 int scount = 0;
-                expand_stage<kLearned>(P, W, L, C, L.node[0], 0, true, scount);
+                expand_stage<kLearned, kServe>(P, W, L, C, L.node[0], 0, true, scount);
                 if (scount && !W.stop) {
                     wave_sync();
-                    flush_stage<kLearned>(P, W, L, scount, (int)C.n, (int)P.spi_words, (int)C.triple_count, true);
+                    flush_stage<kLearned, kServe>(P, W, L, C, scount, (int)C.n, (int)P.spi_words, (int)C.triple_count, true);
                 }
                 PHASE_ADD(L, kPhaseRoot);
             }
+            if (kServe && !W.stop) serve_flush(P, W, L);
             continue;
         } else if (P.donate) {
             PHASE_T0();
@@ -1429,7 +1735,7 @@ __device__ __forceinline__ void search_body(const SearchParams &P) {
                         const unsigned long long h2 = ctl_load(P, kCtlPoolHead);
                         complete_after();
                         const unsigned long long t2 = ctl_load(P, kCtlPoolTail);
-                        done = (idl == reg && h2 == h0 && t2 == t0) ? 1 : 0;
+                        done = (!kServe && idl == reg && h2 == h0 && t2 == t0) ? 1 : 0;
                     }
                     if (work) {
                         // leave the idle set before taking: clearing the own
@@ -1447,7 +1753,7 @@ __device__ __forceinline__ void search_body(const SearchParams &P) {
                     have = take_one(P, W, L);
                     PHASE_ADD(L, kPhaseShare);
                     if (!have) continue;
-                } else if (should_stop(P)) {
+                } else if (should_stop<kServe>(P)) {
                     break;
                 }
             }
@@ -1465,14 +1771,23 @@ __device__ __forceinline__ void search_body(const SearchParams &P) {
         // ---- expand the nodes in L.node[0..batch) (one configuration): their
         // children are staged together and deduplicated in one pass
         const int cfg = uniform((int)L.node[0][0]);
-        const ConfigDesc &C = P.cfgs[cfg];
+        const ConfigDesc &C = kServe ? cfg_copy[wib] : P.cfgs[cfg];
         {
             PHASE_T0();
-            switch_config(P, W, L, cfg);
-            lds_add(&L.live, (unsigned long long)(-(long long)batch));   // the nodes just taken
-            const bool skip = config_found(P, cfg);
+            bool skip;
+            if constexpr (kServe) {
+                skip = serve_switch(P, W, L, cfg_copy[wib], cfg);
+                lds_add(&L.takes, (unsigned long long)batch);   // the nodes just taken
+            } else {
+                switch_config(P, W, L, cfg);
+                lds_add(&L.live, (unsigned long long)(-(long long)batch));   // the nodes just taken
+                skip = config_found(P, cfg);
+            }
             PHASE_ADD(L, kPhaseCheck);
-            if (skip) continue;
+            if (skip) {
+                if (kServe) serve_flush(P, W, L);   // dropped: taken without children
+                continue;
+            }
         }
         if (W.lane < kNumCounters) L.pend[W.lane] = 0;
         wave_sync();
@@ -1491,7 +1806,7 @@ __device__ __forceinline__ void search_body(const SearchParams &P) {
             const int d = uniform((int)(nd[1] & 0xFF));
             const int e = uniform((int)((nd[1] >> 8) & 0xFF));
             if ((C.accept[d] >> e) & 1ULL) {
-                report_found(P, W, nd, cfg, d, -1);
+                report_found<kServe>(P, W, nd, cfg, d, -1);
                 terminal = true;
                 break;
             }
@@ -1499,9 +1814,9 @@ __device__ __forceinline__ void search_body(const SearchParams &P) {
             lds_add(&L.cnt[kCntExpansions], 1);
             lds_add(&L.cnt[kCntGenerated], C.triple_count);
             ++expanded;
-            const int goal = expand_stage<kLearned>(P, W, L, C, nd, b, false, scount);
+            const int goal = expand_stage<kLearned, kServe>(P, W, L, C, nd, b, false, scount);
             if (goal >= 0) {
-                report_found(P, W, nd, cfg, d + 1, goal);
+                report_found<kServe>(P, W, nd, cfg, d + 1, goal);
                 terminal = true;   // the configuration is found: staged children are dropped
                 break;
             }
@@ -1513,14 +1828,17 @@ __device__ __forceinline__ void search_body(const SearchParams &P) {
         if (!terminal && !W.stop) {
             if (scount) {
                 wave_sync();
-                flush_stage<kLearned>(P, W, L, scount, (int)C.n, (int)P.spi_words, (int)C.triple_count, false);
+                flush_stage<kLearned, kServe>(P, W, L, C, scount, (int)C.n, (int)P.spi_words, (int)C.triple_count, false);
             }
             // nodes of the batch not expanded return on top, the top one last
-            for (int b = rest - 1; b >= 0 && !W.stop; --b) push_node(P, W, L, L.node[b]);
+            for (int b = rest - 1; b >= 0 && !W.stop; --b) push_node<kServe>(P, W, L, L.node[b]);
             wave_sync();
             if (W.lane < kNumCounters) L.cnt[W.lane] += L.pend[W.lane];
             wave_sync();
         }
+        // serving: the batch's counters and its pushes minus its takes (a
+        // terminal batch's unexpanded nodes and staged children are dropped)
+        if (kServe && !W.stop) serve_flush(P, W, L);
 
         // donation: every share_mask + 1 expansions, a wave with at least two
         // nodes hands up to half of them (at most share_max) to an idle wave
@@ -1534,7 +1852,7 @@ __device__ __forceinline__ void search_body(const SearchParams &P) {
             PHASE_ADD(L, kPhaseShare);
         }
     }
-    flush_counters(P, W, L);
+    flush_counters<kServe>(P, W, L);
 #ifdef TDP_PHASE_TIMING
     if (W.lane < kNumPhases) fetch_add(&P.ctrl[kCtlPhase0 + W.lane].v, L.phase[W.lane]);
 #endif
@@ -1557,10 +1875,16 @@ __device__ __forceinline__ void search_body(const SearchParams &P) {
 #define TDP_WAVES_PER_EU 3
 #endif
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TDP_WAVES_PER_EU))) void search_kernel(SearchParams P) {
-    search_body<false>(P);
+    search_body<false, false>(P);
 }
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TDP_WAVES_PER_EU))) void search_learned_kernel(SearchParams P) {
-    search_body<true>(P);
+    search_body<true, false>(P);
+}
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TDP_WAVES_PER_EU))) void serve_kernel(SearchParams P) {
+    search_body<false, true>(P);
+}
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TDP_WAVES_PER_EU))) void serve_learned_kernel(SearchParams P) {
+    search_body<true, true>(P);
 }
 
 // Reset of the launch state (replaces the per-search memsets of

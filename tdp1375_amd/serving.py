@@ -1,40 +1,36 @@
-"""Serving loop: single-configuration search requests over concurrent slots.
+"""Serving: single-configuration search requests through the device-resident
+request loop (tdp_serve_*).
 
 The reference serves the proof generator's ForkJoinPool: every worker thread
-takes a slot from a pool, runs one search on it and gives it back
-(GPUSortingSearch.java:19-28,47-87).  Here the same slots run continuous
-batches:
+takes a slot, runs one blocking search and gives the slot back, one launch per
+call (GPUSortingSearch.java:19-28,47-87, SortOrExtend.java:96-104).  Here one
+persistent launch serves every caller: its waves take requests as they arrive
+(continuous batching, no launch boundaries), hand work to idle waves, share
+the queue memory and the dedup table, and reply through pinned host memory;
+a request waits for admission only while no device slot is free or its root
+nodes would not fit in the queue memory beside the requests in flight.
 
-* requests go into one admission queue;
-* every slot has a worker thread; a worker whose slot is free takes every
-  request waiting at that moment (up to max_batch) and runs them as one
-  persistent launch (tdp_search_batch), so requests that arrive while other
-  launches run are admitted as soon as a slot frees up;
-* the slots' launches run concurrently on the GPU on their own streams, each
-  with a share of the compute units (waves per CU / slots) and of the queue
-  memory (the queue budget is split evenly over the slots at init), and every
-  slot keeps its device buffers for the whole session.
-
-A request's verdict does not depend on what it is batched with: batching only
-changes which wave takes which node first.  Latency is measured from arrival
-to the end of the launch that served the request.
+DeviceServer wraps a session.  Its submit() hands a request to a pool of
+caller threads (the ForkJoin workers' stand-in), each of which blocks in
+tdp_serve_search until its own reply, and returns a Future.  A request's
+verdict does not depend on what it runs beside: that only changes which wave
+takes which node first.
 """
 from __future__ import annotations
 
 import argparse
+import ctypes
 import json
-import queue
 import sys
 import threading
 import time
-from concurrent.futures import Future
+from concurrent.futures import Future, ThreadPoolExecutor
 from dataclasses import dataclass, field
-from typing import List, Optional, Sequence
+from typing import List, Sequence
 
 import numpy as np
 
 from . import _lib
-from . import gpu_sorting_search as G
 from . import workloads as W
 
 
@@ -46,113 +42,77 @@ class Request:
     min_rate: float
     max_depth: int
     arrival: float = 0.0
-    future: Future = field(default_factory=Future)
 
 
 @dataclass
 class Reply:
     found: bool
     moves: list            # positions (i, j, k) per move
-    expansions: int
-    latency: float         # seconds from arrival to completion
-    batch: int             # size of the launch that served it
-    slot: int
+    expansions: int        # of the batches ended before the verdict
+    latency: float         # seconds from arrival to the verdict
 
 
-class SearchServer:
-    def __init__(self, slots: int = 8, queue_bytes_total: int = 8 << 30, dedup_per_slot: int = 1 << 22,
-                 max_batch: int = 256, learned: bool = False, waves_per_cu: int = 0):
-        self.slots = slots
-        self.max_batch = max_batch
-        self.flags = _lib.FLAG_LEARNED if learned else 0
-        G.init_slots(slots, 1, queue_bytes_total // slots, dedup_per_slot)
-        # a share of the compute units per slot so that the launches overlap;
-        # small launches of single requests do better with hand-offs of 4
-        # nodes than with the default 8 (measured: 4,721 against 3,450 req/s
-        # at saturation)
-        G.set_tuning(waves_per_cu=waves_per_cu or max(1, 12 // slots), share_max=4)
-        self._q: "queue.Queue[Optional[Request]]" = queue.Queue()
-        self._stop = False
-        self.launches = 0
+class DeviceServer:
+    """A device-resident serving session on the current GPU."""
+
+    def __init__(self, queue_bytes: int = 16 << 30, dedup: int = 1 << 26, slots: int = 256, max_n: int = 63,
+                 learned: bool = False, lifetime_s: int = 600, callers: int = 0):
+        self.lib = _lib.load()
+        flags = _lib.FLAG_LEARNED if learned else 0
+        _lib.check(self.lib.tdp_serve_start(queue_bytes, dedup, slots, max_n, flags, lifetime_s))
+        # one caller thread per request the session admits at once
+        self._pool = ThreadPoolExecutor(max_workers=callers or slots, thread_name_prefix="serve-caller")
+        self._closed = False
         self._lock = threading.Lock()
-        self._workers = [threading.Thread(target=self._work, args=(s,), daemon=True) for s in range(slots)]
-        for w in self._workers:
-            w.start()
+
+    def search(self, pi, spi, e0: int, min_rate: float, max_depth: int, arrival: float = 0.0) -> Reply:
+        """One blocking request (the caller thread waits in the library)."""
+        n = len(pi)
+        moves = (ctypes.c_int32 * (_lib.MAX_DEPTH * 3))()
+        depth = ctypes.c_int32(0)
+        cnt = _lib.Counters()
+        t0 = arrival or time.perf_counter()
+        st = self.lib.tdp_serve_search(_lib.u8(pi, 64), _lib.u8(spi, 64), n, e0, ctypes.c_float(min_rate),
+                                       max_depth, 0, moves, ctypes.byref(depth), ctypes.byref(cnt))
+        done = time.perf_counter()
+        _lib.check(st)
+        d = depth.value
+        pos = [(moves[3 * m], moves[3 * m + 1], moves[3 * m + 2]) for m in range(d)]
+        return Reply(d > 0, pos, int(cnt.expansions), done - t0)
 
     def submit(self, req: Request) -> Future:
         if not req.arrival:
             req.arrival = time.perf_counter()
-        self._q.put(req)
-        return req.future
+        return self._pool.submit(self.search, req.pi, req.spi, req.even_cycles, req.min_rate, req.max_depth,
+                                 req.arrival)
 
-    def _work(self, slot: int) -> None:
-        while True:
-            first = self._q.get()
-            if first is None:
-                self._q.put(None)   # the stop mark stays for the other workers
-                return
-            batch = [first]
-            while len(batch) < self.max_batch:
-                try:
-                    r = self._q.get_nowait()
-                except queue.Empty:
-                    break
-                if r is None:
-                    self._q.put(None)   # leave the stop mark for the other workers
-                    break
-                batch.append(r)
-            try:
-                res = G.search_batch(slot, [r.pi for r in batch], [r.spi for r in batch],
-                                     [r.even_cycles for r in batch], [r.min_rate for r in batch],
-                                     [r.max_depth for r in batch], 0, flags=self.flags)
-                done = time.perf_counter()
-                with self._lock:
-                    self.launches += 1
-                for c, r in enumerate(batch):
-                    r.future.set_result(Reply(bool(res.found[c]), res.moves[c], int(res.counters["expansions"][c]),
-                                              done - r.arrival, len(batch), slot))
-            except BaseException as exc:   # every request of a failed launch gets the error
-                for r in batch:
-                    if not r.future.done():
-                        r.future.set_exception(exc)
+    def stats(self) -> dict:
+        c, f, r = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_int64()
+        _lib.check(self.lib.tdp_serve_stats(ctypes.byref(c), ctypes.byref(f), ctypes.byref(r)))
+        return {"completed": c.value, "in_flight": f.value, "reserved_bytes": r.value}
 
     def close(self) -> None:
-        self._q.put(None)
-        for w in self._workers:
-            w.join()
-        G.set_tuning()
-        G.destroy_slots()
+        with self._lock:
+            if self._closed:
+                return
+            self._closed = True
+        self._pool.shutdown(wait=True)
+        _lib.check(self.lib.tdp_serve_stop())
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
 
 
-def request_pool(count: int, seed: int, max_depth: int = 11, max_ratio: float = 1.375,
-                 max_expansions: int = 200000):
-    """Configurations with n uniform in 9..48 (the depth-11 generator's odd
-    cycles), split by the plain search's verdict into satisfiable and
-    unsatisfiable ones by a batched plain search in a session of its own (call
-    it before a SearchServer exists).  Configurations that took more than
-    max_expansions to decide are left out, so that a request costs
-    milliseconds rather than seconds."""
-    cases = W.odd_cycle_mix(count, seed, 9, 48)
-    rate = W.Workload("pool", cases, max_ratio, max_depth, 0, 0, 1).min_rate
-    G.init_slots(1, 1, 16 << 30, 1 << 26)
-    try:
-        # configurations still open after 2 s of a launch are left out
-        G.set_launch_deadline(2000)
-        res = G.search_batch(0, [c.pi for c in cases], [c.spi for c in cases], [c.even_cycles for c in cases],
-                             rate, max_depth, 0, flags=_lib.FLAG_SOFT_DEADLINE)
-    finally:
-        G.destroy_slots()
-    light = (res.counters["expansions"] <= max_expansions) & (res.depth >= 0)
-    sat = [c for c, f, ok in zip(cases, res.found, light) if f and ok]
-    unsat = [c for c, f, ok in zip(cases, res.found, light) if not f and ok]
-    return sat, unsat, rate
-
-
-def poisson_run(server: SearchServer, pool: List, rate_per_s: float, duration_s: float, min_rate: float,
+def poisson_run(server: DeviceServer, pool: List, rate_per_s: float, duration_s: float, min_rate: float,
                 max_depth: int, seed: int) -> dict:
-    """Poisson arrivals for duration_s; returns throughput and latency percentiles."""
+    """Poisson arrivals for duration_s; returns throughput and latency
+    percentiles (arrival to verdict) and the replies in arrival order with
+    the pool index of each request."""
     rng = np.random.default_rng(seed)
-    futures = []
+    futures, picks = [], []
     t0 = time.perf_counter()
     t_next = t0
     while True:
@@ -162,7 +122,9 @@ def poisson_run(server: SearchServer, pool: List, rate_per_s: float, duration_s:
         now = time.perf_counter()
         if t_next > now:
             time.sleep(t_next - now)
-        c = pool[int(rng.integers(len(pool)))]
+        k = int(rng.integers(len(pool)))
+        c = pool[k]
+        picks.append(k)
         futures.append(server.submit(Request(c.pi, c.spi, c.even_cycles, min_rate, max_depth, arrival=t_next)))
     replies = [f.result() for f in futures]
     wall = time.perf_counter() - t0
@@ -170,33 +132,41 @@ def poisson_run(server: SearchServer, pool: List, rate_per_s: float, duration_s:
     return {"offered_rate": rate_per_s, "requests": len(replies), "throughput": len(replies) / wall,
             "p50_ms": float(np.percentile(lat, 50)) if len(lat) else None,
             "p99_ms": float(np.percentile(lat, 99)) if len(lat) else None,
-            "mean_batch": float(np.mean([r.batch for r in replies])) if replies else None,
-            "found": int(sum(r.found for r in replies))}
+            "found": int(sum(r.found for r in replies)),
+            "_replies": replies, "_picks": picks}
+
+
+def pool_cases(count: int = 1024, seed: int = 5) -> List[W.Case]:
+    """The request generator: n uniform in 9..48, the depth-11 generator's
+    odd cycles (3..12 cycles of lengths 3/5/7, 30 % with an oriented 3-cycle)."""
+    return W.odd_cycle_mix(count, seed, 9, 48)
 
 
 def main(argv=None):
-    ap = argparse.ArgumentParser(description="Poisson serving sweep over concurrent slots")
+    import os
+    ap = argparse.ArgumentParser(description="Poisson serving sweep through the device-resident loop")
     ap.add_argument("--rates", default="10,100,1000,10000")
     ap.add_argument("--duration", type=float, default=4.0)
-    ap.add_argument("--slots", type=int, default=8)
-    ap.add_argument("--pool", type=int, default=4096)
     ap.add_argument("--ordering", choices=("plain", "learned"), default="plain")
     ap.add_argument("--model", default=None)
+    ap.add_argument("--pool", default=os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                                                   "tests", "golden", "serving_pool_n9-48-d11.json"),
+                    help="labelled request pool (indices into pool_cases and their verdicts)")
     args = ap.parse_args(argv)
-    sat, unsat, min_rate = request_pool(args.pool, 5)
-    server = SearchServer(slots=args.slots, learned=args.ordering == "learned")
+    with open(args.pool) as fh:
+        doc = json.load(fh)
+    cases = pool_cases(doc["generated"], doc["seed"])
+    pool = [cases[i] for i in doc["satisfiable"]] + [cases[i] for i in doc["unsatisfiable"]]
+    rate = float(np.float32(2.0) / np.float32(doc["max_ratio"]))
     if args.ordering == "learned":
-        import os
         from . import train_ordering
         train_ordering.load(args.model or os.path.join(os.path.dirname(__file__), "data", "ordering_mlp_d11.npz"))
-    k = min(len(sat), len(unsat))
-    pool = sat[:k] + unsat[:k]   # half satisfiable within depth 11
-    print(f"pool: {len(sat)} satisfiable, {len(unsat)} not; using {2 * k}", file=sys.stderr, flush=True)
-    for rate in [float(x) for x in args.rates.split(",")]:
-        out = poisson_run(server, pool, rate, args.duration, min_rate, 11, seed=int(rate))
-        out.update({"ordering": args.ordering, "slots": args.slots})
-        print(json.dumps(out), flush=True)
-    server.close()
+    with DeviceServer(learned=args.ordering == "learned", max_n=48) as server:
+        for r in [float(x) for x in args.rates.split(",")]:
+            out = poisson_run(server, pool, r, args.duration, rate, doc["max_depth"], seed=int(r))
+            out = {k: v for k, v in out.items() if not k.startswith("_")}
+            out.update({"ordering": args.ordering})
+            print(json.dumps(out), flush=True)
 
 
 if __name__ == "__main__":

@@ -41,3 +41,16 @@ def assert_witness(pi, spi, e0, min_rate, max_depth, positions, full=False):
     assert e >= 0, "invalid move in witness"
     assert oracle.satisfies_rate(e, len(positions), len(pi), e0, min_rate, full), (
         f"witness does not satisfy the rate: gain {e - e0} in {len(positions)} moves")
+
+
+def load_serving_pool():
+    """(satisfiable cases, unsatisfiable cases, float32 min_rate, fixture) of
+    the oracle-labelled serving pool (serving_pool_n9-48-d11.json)."""
+    from tdp1375_amd import serving as S
+    with open(os.path.join(GOLDEN, "serving_pool_n9-48-d11.json")) as fh:
+        doc = json.load(fh)
+    cases = S.pool_cases(doc["generated"], doc["seed"])
+    sat = [cases[i] for i in doc["satisfiable"]]
+    unsat = [cases[i] for i in doc["unsatisfiable"]]
+    rate = float(np.float32(2.0) / np.float32(doc["max_ratio"]))
+    return sat, unsat, rate, doc

@@ -22,7 +22,8 @@ JINT = ctypes.c_int32
 class FakeJNIEnv:
     IDX = dict(FindClass=6, ThrowNew=14, ExceptionClear=17, DeleteLocalRef=23, GetArrayLength=171,
                NewObjectArray=172, SetObjectArrayElement=174, NewByteArray=176, NewIntArray=179,
-               GetByteArrayRegion=200, SetByteArrayRegion=208, SetIntArrayRegion=211, ExceptionCheck=228)
+               GetByteArrayRegion=200, GetIntArrayRegion=203, SetByteArrayRegion=208, SetIntArrayRegion=211,
+               SetFloatArrayRegion=213, ExceptionCheck=228)
 
     def __init__(self):
         self.objects = {}
@@ -48,6 +49,9 @@ class FakeJNIEnv:
         cb("SetByteArrayRegion", ctypes.CFUNCTYPE(None, VP, VP, JINT, JINT, ctypes.POINTER(ctypes.c_int8)),
            self._set_region)
         cb("SetIntArrayRegion", ctypes.CFUNCTYPE(None, VP, VP, JINT, JINT, ctypes.POINTER(JINT)),
+           self._set_region)
+        cb("GetIntArrayRegion", ctypes.CFUNCTYPE(None, VP, VP, JINT, JINT, ctypes.POINTER(JINT)), self._get_ints)
+        cb("SetFloatArrayRegion", ctypes.CFUNCTYPE(None, VP, VP, JINT, JINT, ctypes.POINTER(ctypes.c_float)),
            self._set_region)
         self.env_struct = (VP * 1)(ctypes.cast(self.table, VP))
         self.env = ctypes.cast(self.env_struct, VP)
@@ -79,13 +83,24 @@ class FakeJNIEnv:
         for q in range(n):
             buf[q] = ctypes.c_int8(data[start + q]).value
 
+    def _get_ints(self, env, arr, start, n, buf):
+        data = self.objects[arr][1]
+        for q in range(n):
+            buf[q] = int(data[start + q])
+
     def _set_region(self, env, arr, start, n, buf):
         data = self.objects[arr][1]
         for q in range(n):
-            data[start + q] = int(buf[q])
+            data[start + q] = buf[q]
 
     def byte_array(self, values):
         return self._new(("bytes", [ctypes.c_int8(v & 0xFF).value for v in values]))
+
+    def int_array(self, values):
+        return self._new(("ints", [int(v) for v in values]))
+
+    def float_array(self, n):
+        return self._new(("floats", [0.0] * n))
 
     def value(self, h):
         kind, data = self.objects[h]
@@ -115,6 +130,26 @@ def natives():
     filt.argtypes = [VP, VP, VP, JINT, VP, JINT, JINT, ctypes.c_uint8]
     filt.restype = VP
     return init, destroy, search, cancel, filt
+
+
+def batch_natives():
+    lib = T.load()
+    pre = "Java_br_unb_cic_tdp_base_GPUSortingSearch_"
+    batch = getattr(lib, pre + "searchBatch")
+    batch.argtypes = [VP, VP, VP, VP, VP, VP, ctypes.c_float, JINT, ctypes.c_uint8, JINT, JINT]
+    batch.restype = VP
+    canon = getattr(lib, pre + "canonicalBatch")
+    canon.argtypes = [VP, VP, VP, VP, VP, VP, VP, VP, VP]
+    canon.restype = JINT
+    return batch, canon
+
+
+def _rows(cases, attr):
+    out = []
+    for c in cases:
+        row = list(getattr(c, attr)) + [0] * (64 - c.n)
+        out.extend(row)
+    return out
 
 
 def test_errors_become_runtime_exceptions():
@@ -209,3 +244,134 @@ def test_jni_out_of_memory_becomes_runtime_exception(gpu_lib):
         assert env.exception is None and r is not None
     finally:
         destroy(env.env, None)
+
+
+@pytest.mark.gpu
+def test_jni_concurrent_callers_share_the_serving_session(gpu_lib, monkeypatch):
+    """TDP_SERVE=1: initSlots starts one serving session holding the slots'
+    queue memory, and searchForSortingSeq calls from eight threads (one slot
+    and one JNIEnv each, as ForkJoin workers) share it.  Every result equals
+    the oracle's label, in the mix and alone (one call at a time from one
+    thread); per-call latency of the JNI path is printed."""
+    from helpers import load_serving_pool
+    import threading
+    import time
+    sat, unsat, rate, doc = load_serving_pool()
+    pool = sat[:48] + unsat[:48]
+    labels = [True] * 48 + [False] * 48
+    monkeypatch.setenv("TDP_SERVE", "1")
+    monkeypatch.setenv("TDP_SERVE_LIFETIME_S", "300")
+    init, destroy, search, cancel, filt = natives()
+    env0 = FakeJNIEnv()
+    init(env0.env, None, 8, 1, 1 << 30, 1 << 22)
+    assert env0.exception is None
+    try:
+        def call(env, c, slot):
+            r = search(env.env, None, env.byte_array(c.pi), env.byte_array(c.spi), c.even_cycles, rate,
+                       doc["max_depth"], 0, slot)
+            assert env.exception is None and r is not None, env.exception
+            return [tuple(m) for m in env.value(r)]
+
+        results = [None] * len(pool)
+        lat = [0.0] * len(pool)
+        errors = []
+
+        def worker(t):
+            env = FakeJNIEnv()
+            try:
+                for q in range(t, len(pool), 8):
+                    t0 = time.perf_counter()
+                    results[q] = call(env, pool[q], t)
+                    lat[q] = time.perf_counter() - t0
+            except BaseException as e:  # noqa: BLE001
+                errors.append(e)
+
+        threads = [threading.Thread(target=worker, args=(t,)) for t in range(8)]
+        for th in threads:
+            th.start()
+        for th in threads:
+            th.join(240)
+        assert not errors, errors
+        assert [bool(r) for r in results] == labels
+        for c, pos in zip(pool, results):
+            if pos:
+                e, _, _ = oracle.replay_positions(c.pi, c.spi, pos)
+                assert oracle.satisfies_rate(e, len(pos), c.n, c.even_cycles, rate, False)
+        alone = [bool(call(env0, c, 0)) for c in pool[::3]]
+        assert alone == labels[::3]
+        ms = np.array(lat) * 1e3
+        print(f"JNI path, 8 threads: p50 {np.percentile(ms, 50):.2f} ms, p99 {np.percentile(ms, 99):.2f} ms",
+              flush=True)
+    finally:
+        destroy(env0.env, None)
+    assert env0.exception is None
+
+
+def test_batch_natives_reject_bad_arguments_without_a_gpu():
+    """searchBatch / canonicalBatch: size mismatches, an invalid slot and a
+    configuration with a fixed point become RuntimeExceptions (checked before
+    any device work)."""
+    batch, canon = batch_natives()
+    env = FakeJNIEnv()
+    wl = W.workload_random_n12()
+    cases = wl.cases[:3]
+    r = batch(env.env, None, env.byte_array(_rows(cases, "pi")), env.byte_array(_rows(cases, "spi")[:64]),
+              env.int_array([12] * 3), env.int_array([c.even_cycles for c in cases]), wl.min_rate, 6, 0, 0, 0)
+    assert r is None and env.exception[1] == "performGpuSortingSearch: inconsistent input sizes"
+    env.exception = None
+    r = batch(env.env, None, env.byte_array(_rows(cases, "pi")), env.byte_array(_rows(cases, "spi")),
+              env.int_array([12] * 3), env.int_array([c.even_cycles for c in cases]), wl.min_rate, 6, 0, 0, 9)
+    assert r is None and env.exception == ("java/lang/RuntimeException", "invalid slot")
+    env.exception = None
+    spi = list(range(5)) + [0] * 59     # identity: every symbol a fixed point
+    pi = list(range(5)) + [0] * 59
+    got = canon(env.env, None, env.byte_array(spi), env.byte_array(pi), env.int_array([5]), env.float_array(64),
+                env.int_array([0]), env.int_array([0]), env.byte_array([0] * 64))
+    assert got == 0 and env.exception == ("java/lang/RuntimeException", "spi has a fixed point")
+
+
+@pytest.mark.gpu
+def test_batch_natives_round_trip_on_gpu(gpu_lib):
+    """searchBatch equals the oracle (deterministic mode: verdicts and
+    sequences) and canonicalBatch reproduces ConfigurationTest's canonical
+    signatures and hash codes."""
+    import json
+    import os
+    from oracle import canon_ref as C
+    batch, canon = batch_natives()
+    init, destroy, search, cancel, filt = natives()
+    env = FakeJNIEnv()
+    init(env.env, None, 1, 1, 256 << 20, 1 << 20)
+    try:
+        wl = W.workload_random_n12()
+        cases = wl.cases[:64]
+        r = batch(env.env, None, env.byte_array(_rows(cases, "pi")), env.byte_array(_rows(cases, "spi")),
+                  env.int_array([c.n for c in cases]), env.int_array([c.even_cycles for c in cases]), wl.min_rate,
+                  wl.max_depth, 0, T.FLAG_DETERMINISTIC, 0)
+        assert env.exception is None and r is not None
+        out = np.array(env.value(r), np.int64).reshape(len(cases), 47)
+        for c, case in enumerate(cases):
+            want = oracle.search(case.pi, case.spi, case.even_cycles, wl.min_rate, wl.max_depth)
+            assert bool(out[c, 0]) == want.found
+            assert int(out[c, 1]) == want.depth
+            assert [tuple(out[c, 2 + 3 * m: 5 + 3 * m]) for m in range(want.depth)] == want.moves
+    finally:
+        destroy(env.env, None)
+    with open(os.path.join(os.path.dirname(__file__), "golden", "configuration_vectors.json")) as fh:
+        gold = json.load(fh)["configurations"]
+    spis, pis, ns = [], [], []
+    for g in gold:
+        cyc = C.parse_cycles(g["spi"])
+        n = sum(len(c) for c in cyc)
+        spi = W.one_line(cyc, n)
+        spis += spi + [0] * (64 - n)
+        pis += list(range(n)) + [0] * (64 - n)
+        ns.append(n)
+    sig, hashes, index, cs = env.float_array(64 * len(ns)), env.int_array([0] * len(ns)), \
+        env.int_array([0] * len(ns)), env.byte_array([0] * 64 * len(ns))
+    got = canon(env.env, None, env.byte_array(spis), env.byte_array(pis), env.int_array(ns), sig, hashes, index, cs)
+    assert env.exception is None and got == len(ns)
+    sigs = np.array(env.value(sig), np.float32).reshape(len(ns), 64)
+    for q, g in enumerate(gold):
+        assert env.value(hashes)[q] == g["hash"]
+        assert np.array_equal(sigs[q, :ns[q]], np.asarray(g["canonical"], np.float32))
