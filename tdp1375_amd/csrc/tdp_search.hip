@@ -526,12 +526,16 @@ __device__ __forceinline__ bool receive_nodes(const SearchParams &P, WaveState &
 // ---- spill pool: a bounded MPMC ring (per-slot sequence numbers) that takes
 // the oldest nodes of a full stack when no wave is idle; idle waves drain it.
 
-// Moves the oldest node of the private stack into the spill pool.  Returns
-// false when the pool is full.
-__device__ __forceinline__ bool donate_one(const SearchParams &P, WaveState &W) {
-    if (!P.pool_seq) return false;
+// Moves up to `most` of the oldest nodes of the private stack into the
+// spill pool with one reservation of consecutive tickets on the tail (one
+// CAS for the whole group, so waves spilling at once do not serialise node by
+// node).  Each ticket's slot is waited for (its previous node taken), filled,
+// and published with the ticket's sequence number.  Returns the number of
+// nodes moved (0: the pool is full).
+__device__ __forceinline__ uint32_t donate_nodes(const SearchParams &P, WaveState &W, uint32_t most) {
+    if (!P.pool_seq || most == 0) return 0;
     unsigned long long ticket = 0;
-    int ok = 0;
+    uint32_t got = 0;
     if (W.lane == 0) {
         // a lost CAS only means another wave moved the tail: retry until the
         // pool is really full (or the watchdog fires)
@@ -542,36 +546,50 @@ __device__ __forceinline__ bool donate_one(const SearchParams &P, WaveState &W) 
             const unsigned long long h = ctl_load(P, kCtlPoolHead);
             complete_after();
             const unsigned long long t = ctl_load(P, kCtlPoolTail);
-            if (t >= h && t - h >= P.pool_cap) break;
-            if (cas64(&P.ctrl[kCtlPoolTail].v, t, t + 1)) { ticket = t; ok = 1; break; }
+            const unsigned long long used = t >= h ? t - h : P.pool_cap;
+            if (used >= P.pool_cap) break;
+            // at most an eighth of the free room (one node at least): a
+            // nearly full pool is shared out in small spills, not taken
+            // whole by the first waves to overflow
+            const unsigned long long room = P.pool_cap - used;
+            const unsigned long long share = room >= 8 ? room >> 3 : 1;
+            const uint32_t take = (uint32_t)(share < most ? share : most);
+            if (cas64(&P.ctrl[kCtlPoolTail].v, t, t + take)) {
+                ticket = t;
+                got = take;
+                break;
+            }
             if ((attempt & 63) == 63 && past_deadline(W)) break;
             __builtin_amdgcn_s_sleep(1);
         }
     }
-    ok = __shfl(ok, 0);
-    if (!ok) return false;
+    got = (uint32_t)__shfl((int)got, 0);
+    if (got == 0) return 0;
     ticket = ((unsigned long long)(unsigned)__shfl((int)(ticket >> 32), 0) << 32) |
              (unsigned)__shfl((int)(ticket & 0xFFFFFFFFu), 0);
-    const size_t slot = (size_t)(ticket & (P.pool_cap - 1));
-    int late = 0;
-    if (W.lane == 0) {
-        while (ld_relaxed(&P.pool_seq[slot]) != ticket) {
-            if (past_deadline(W)) { late = 1; break; }
-            __builtin_amdgcn_s_sleep(2);
+    for (uint32_t g = 0; g < got; ++g) {
+        const size_t slot = (size_t)((ticket + g) & (P.pool_cap - 1));
+        int late = 0;
+        if (W.lane == 0) {
+            while (ld_relaxed(&P.pool_seq[slot]) != ticket + g) {
+                if (past_deadline(W)) { late = 1; break; }
+                __builtin_amdgcn_s_sleep(2);
+            }
         }
+        if (__shfl(late, 0)) { set_watchdog(P, W); return 0; }
+        complete_after();
+        const uint32_t *src = stack_slot(P, W, W.lo + g);
+        uint32_t *dst = P.pool_nodes + slot * P.node_words;
+        if (W.lane < (int)P.node_words) st32(dst + W.lane, src[W.lane]);
     }
-    if (__shfl(late, 0)) { set_watchdog(P, W); return false; }
-    complete_after();
-    const uint32_t *src = stack_slot(P, W, W.lo);
-    uint32_t *dst = P.pool_nodes + slot * P.node_words;
-    if (W.lane < (int)P.node_words) st32(dst + W.lane, src[W.lane]);
     complete_before();
-    if (W.lane == 0) {
-        st64(&P.pool_seq[slot], ticket + 1);
-        fetch_add(&P.ctrl[kCtlSpills].v, 1ULL);
+    if ((uint32_t)W.lane < got) {
+        const unsigned long long tk = ticket + (unsigned long long)W.lane;
+        st64(&P.pool_seq[(size_t)(tk & (P.pool_cap - 1))], tk + 1);
     }
-    W.lo++;
-    return true;
+    if (W.lane == 0) fetch_add(&P.ctrl[kCtlSpills].v, (unsigned long long)got);
+    W.lo += got;
+    return got;
 }
 
 // Takes one node from the shared pool into L.node.
@@ -624,8 +642,13 @@ __device__ __forceinline__ bool make_room(const SearchParams &P, WaveState &W, W
         uint32_t most = (W.hi - W.lo) / 2;
         if (most > P.share_max) most = P.share_max;
         if (share_nodes(P, W, most, W.hi)) return true;
-        for (int r = 0; r < 32 && W.hi - W.lo > 1; ++r)
-            if (!donate_one(P, W)) break;
+        // the oldest nodes to the pool: an eighth of the stack, 1..32 and
+        // one at least staying.  A larger share of a small stack would
+        // leave the stacks half empty while the pool fills with their halves.
+        uint32_t spill = P.stack_cap >> 3;
+        spill = spill < 1u ? 1u : spill > 32u ? 32u : spill;
+        if (spill > W.hi - W.lo - 1) spill = W.hi - W.lo - 1;
+        (void)donate_nodes(P, W, spill);
         if (W.stop) return false;
         if (W.hi - W.lo < P.stack_cap) return true;
         reason = 2;

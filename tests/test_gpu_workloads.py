@@ -6,8 +6,9 @@
   deterministic mode reproduces the oracle's verdict, sequence and every
   counter of every configuration; the work-sharing mode its verdicts;
 * the 4,096-configuration depth-11 batch exactly as bench.py runs it (tuned,
-  work-sharing, dedup 2^27, 32 GiB of queue memory): every verdict the oracle
-  decided within its expansion budget is equal.
+  work-sharing, dedup 2^27, 32 GiB of queue memory), with the plain and with
+  the learned ordering: every verdict the oracle decided within its expansion
+  budget is equal.
 
 Every returned sequence is replayed to an accepted state.
 """
@@ -104,5 +105,15 @@ def check_depth11(wl, got):
 
 def test_depth11_batch_bench_path_verdicts(gpu_lib):
     wl, got = run_depth11_bench_path()
+    decided = check_depth11(wl, got)
+    assert decided >= 3900
+
+
+def test_depth11_batch_learned_ordering_verdicts(gpu_lib):
+    """The learned ordering (committed model) over the same batch and path:
+    every decided verdict equals the oracle's and every sequence replays."""
+    from tdp1375_amd import _lib, train_ordering
+    train_ordering.load(os.path.join(os.path.dirname(T.__file__), "data", "ordering_mlp_d11.npz"))
+    wl, got = run_depth11_bench_path(flags=_lib.FLAG_LEARNED)
     decided = check_depth11(wl, got)
     assert decided >= 3900
