@@ -57,6 +57,9 @@ struct WaveLds {
     uint8_t posof[64];                       // position of each symbol in pi
     uint32_t stage[kStage];
     uint8_t rowsrc[64];                      // learned ordering: lane of each compacted admitted child
+    float carry_score[kWave];                // learned ordering: the best-scored admitted children of
+    uint32_t carry_st[kWave];                // the batch so far (score, stage entry), pushed at its end
+    uint32_t ncarry;
     uint32_t ckey[16][kWave];                // child spi words of each lane's staged child
     unsigned long long cnt[kNumCounters];    // counters of the current configuration
     unsigned long long pend[kNumCounters];   // counters of the current expansion
@@ -719,6 +722,81 @@ __device__ __forceinline__ void push_node(const SearchParams &P, WaveState &W, W
     lds_add(&L.live, 1ULL);
 }
 
+// Pushes the staged child with stage entry st (wave-uniform) of a node of the
+// batch, its move recomputed from the parent in L.node.
+template <bool kServe>
+__device__ __forceinline__ void push_staged(const SearchParams &P, WaveState &W, WaveLds &L, int n, int sw, int T,
+                                            uint32_t st) {
+    const int i = (int)(st & 63u), j = (int)((st >> 6) & 63u), k = (int)((st >> 12) & 63u);
+    const uint32_t *nd = L.node[st >> 24];
+    const uint8_t *spi = reinterpret_cast<const uint8_t *>(nd + kNodeHeaderWords);
+    const uint8_t *pi = spi + 4 * sw;
+    ChildMove m;
+    m.a = pi[i]; m.b = pi[j]; m.c = pi[k];
+    m.na = spi[m.c]; m.nb = spi[m.a]; m.nc = spi[m.b];
+    push_child<kServe>(P, W, L, nd, n, sw, lex_index(n, T, i, j, k), i, j, k, (int)((st >> 18) & 63u), m);
+}
+
+// Learned ordering over a whole batch.  The admitted children of one chunk of
+// the stage (lane l: score, stage entry, when adm) join the children held back
+// so far (L.carry_*); all but the `keep` best of them are pushed in ascending
+// score (ties: chunk before held back, then lane), and the best `keep` are
+// held back again.  keep = kWave between chunks and flushes of a batch, 0 at
+// its end, so the best kWave children of the batch end on top of the stack
+// in score order wherever the stage was cut into chunks and flushes.
+template <bool kServe>
+__device__ __forceinline__ void learned_push(const SearchParams &P, WaveState &W, WaveLds &L, int n, int sw, int T,
+                                             bool adm, float score, uint32_t st, int keep) {
+    const int nc = (int)L.ncarry;
+    bool va = adm;
+    bool vb = W.lane < nc;
+    const float sb = vb ? L.carry_score[W.lane] : 0.f;
+    const uint32_t stb = vb ? L.carry_st[W.lane] : 0u;
+    int total = __popcll(__ballot(va)) + nc;
+    while (total > keep && !W.stop) {
+        // the lowest (score, id) of the two candidates of every lane; ids
+        // 0..63 chunk, 64..127 held back, 128 none
+        float key = va ? score : 3.0e38f;
+        int id = va ? W.lane : 128;
+        if (vb && (sb < key || (sb == key && kWave + W.lane < id))) {
+            key = sb;
+            id = kWave + W.lane;
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            const float ok = __shfl_xor(key, off);
+            const int oi = __shfl_xor(id, off);
+            if (ok < key || (ok == key && oi < id)) {
+                key = ok;
+                id = oi;
+            }
+        }
+        id = uniform(id);
+        const uint32_t from = id < kWave ? st : stb;
+        const uint32_t pst = (uint32_t)__builtin_amdgcn_readlane((int)from, id & (kWave - 1));
+        if (id < kWave) va = va && W.lane != id;
+        else vb = vb && W.lane != id - kWave;
+        --total;
+        push_staged<kServe>(P, W, L, n, sw, T, pst);
+    }
+    wave_sync();   // every lane has read the held-back list before it is rewritten
+    const unsigned long long ma = __ballot(va), mb = __ballot(vb);
+    const int ca = __popcll(ma);
+    const unsigned long long below = (W.lane == 0) ? 0ULL : (~0ULL >> (64 - W.lane));
+    if (va) {
+        const int q = __popcll(ma & below);
+        L.carry_score[q] = score;
+        L.carry_st[q] = st;
+    }
+    if (vb) {
+        const int q = ca + __popcll(mb & below);
+        L.carry_score[q] = sb;
+        L.carry_st[q] = stb;
+    }
+    if (W.lane == 0) L.ncarry = (uint32_t)(ca + __popcll(mb));
+    wave_sync();
+}
+
 // Lane-parallel dedup of staged children (sorting_search.cu:261-298
 // semantics: a child already stored at a depth <= its own is dominated, one
 // stored deeper is admitted and lowers the stored depth, an unknown one is
@@ -954,10 +1032,9 @@ __device__ __forceinline__ void flush_stage(const SearchParams &P, WaveState &W,
         }
         unsigned long long pm = __ballot(admit);
         float score = 0.f;
-        if (kLearned && (pm & (pm - 1))) {
-            // learned ordering: score the admitted children when there are at
-            // least two (MFMA MLP over tiles of 16 of them, compacted in lane
-            // order)
+        if (kLearned && pm) {
+            // learned ordering: score the admitted children (MFMA MLP over
+            // tiles of 16 of them, compacted in lane order)
             PHASE_T0();
             const int e = (int)((nd[1] >> 8) & 0xFF);
             uint32_t f[kFeatWords];
@@ -988,37 +1065,25 @@ __device__ __forceinline__ void flush_stage(const SearchParams &P, WaveState &W,
             }
             PHASE_ADD(L, kPhasePush);
         }
-        {
+        if constexpr (kLearned) {
+            // ascending score over the batch: the best-scored child is pushed
+            // last and so taken first
             PHASE_T0();
-            int bb = kLearned ? 0 : wave_max(admit ? (int)(st >> 24) : -1);
-            // the next child to push, in push order
+            learned_push<kServe>(P, W, L, n, sw, T, admit, score, st, kWave);
+            PHASE_ADD(L, kPhasePush);
+        } else {
+            PHASE_T0();
+            int bb = wave_max(admit ? (int)(st >> 24) : -1);
+            // the next child to push, in push order: children of lower nodes
+            // of the batch first (highest node index), so that the top node's
+            // children end on top
             auto pick = [&]() -> int {
-                int src;
-                if (kLearned) {
-                    // ascending score: the best-scored child is pushed last
-                    // and so taken first (ties: lower lane first)
-                    float key = ((pm >> W.lane) & 1ULL) ? score : 3.0e38f;
-                    int best = W.lane;
-#pragma unroll
-                    for (int off = 32; off > 0; off >>= 1) {
-                        const float ok = __shfl_xor(key, off);
-                        const int ol = __shfl_xor(best, off);
-                        if (ok < key || (ok == key && ol < best)) {
-                            key = ok;
-                            best = ol;
-                        }
-                    }
-                    src = uniform(best);
-                } else {
-                    // children of lower nodes of the batch first (highest node
-                    // index), so that the top node's children end on top
-                    unsigned long long grp = pm & __ballot((int)(st >> 24) == bb);
-                    while (grp == 0) {
-                        --bb;
-                        grp = pm & __ballot((int)(st >> 24) == bb);
-                    }
-                    src = __ffsll((long long)grp) - 1;
+                unsigned long long grp = pm & __ballot((int)(st >> 24) == bb);
+                while (grp == 0) {
+                    --bb;
+                    grp = pm & __ballot((int)(st >> 24) == bb);
                 }
+                const int src = __ffsll((long long)grp) - 1;
                 pm &= ~(1ULL << src);
                 return src;
             };
@@ -1705,11 +1770,15 @@ __device__ __forceinline__ void search_body(const SearchParams &P) {
             if (C.triple_count > 0) {
                 PHASE_T0();
                 int scount = 0;
+                if (kLearned && W.lane == 0) L.ncarry = 0;
+                wave_sync();
                 expand_stage<kLearned, kServe>(P, W, L, C, L.node[0], 0, true, scount);
                 if (scount && !W.stop) {
                     wave_sync();
                     flush_stage<kLearned, kServe>(P, W, L, C, scount, (int)C.n, (int)P.spi_words, (int)C.triple_count, true);
                 }
+                if (kLearned && !W.stop)
+                    learned_push<kServe>(P, W, L, (int)C.n, (int)P.spi_words, (int)C.triple_count, false, 0.f, 0u, 0);
                 PHASE_ADD(L, kPhaseRoot);
             }
             if (kServe && !W.stop) serve_flush(P, W, L);
@@ -1813,6 +1882,7 @@ __device__ __forceinline__ void search_body(const SearchParams &P) {
             }
         }
         if (W.lane < kNumCounters) L.pend[W.lane] = 0;
+        if (kLearned && W.lane == 0) L.ncarry = 0;
         wave_sync();
         int scount = 0, expanded = 0, rest = 0;
         bool terminal = false;
@@ -1845,7 +1915,7 @@ __device__ __forceinline__ void search_body(const SearchParams &P) {
             }
             // half a stage of children (staged, or pushed by a flush inside
             // a wide expansion) ends the batch
-            if (scount + (int)(W.hi - hi0) >= kStage / 2) break;
+            if (scount + (int)(W.hi - hi0) + (kLearned ? (int)L.ncarry : 0) >= kStage / 2) break;
         }
         // a terminal expansion's dedup side effects are not counted
         if (!terminal && !W.stop) {
@@ -1853,6 +1923,8 @@ __device__ __forceinline__ void search_body(const SearchParams &P) {
                 wave_sync();
                 flush_stage<kLearned, kServe>(P, W, L, C, scount, (int)C.n, (int)P.spi_words, (int)C.triple_count, false);
             }
+            if (kLearned && !W.stop)
+                learned_push<kServe>(P, W, L, (int)C.n, (int)P.spi_words, (int)C.triple_count, false, 0.f, 0u, 0);
             // nodes of the batch not expanded return on top, the top one last
             for (int b = rest - 1; b >= 0 && !W.stop; --b) push_node<kServe>(P, W, L, L.node[b]);
             wave_sync();
