@@ -118,8 +118,10 @@ def learned_leg(T, wl, step, verdicts, args, plain_exps_sat):
     one untimed step, then args.learned_steps timed steps; verdicts must equal
     the plain search's."""
     import numpy as np
-    from tdp1375_amd import train_ordering
+    from tdp1375_amd import autotune, train_ordering
     train_ordering.load(args.model)
+    # the learned ordering's own tuning when the cache holds one
+    tuning = (autotune.apply(args.tuning, autotune.cache_key(wl.name, "learned")) if args.tuning else None)
     flags = T._lib.FLAG_LEARNED | (T._lib.FLAG_GRAPH if args.graph else 0)
     r = step(resident=True, extra=flags)
     if not np.array_equal(r.found, verdicts):
@@ -147,7 +149,8 @@ def learned_leg(T, wl, step, verdicts, args, plain_exps_sat):
                                                   "plain_over_learned": float(psat.sum() / max(lsat.sum(), 1.0)),
                                                   "median_learned": float(np.median(lsat)),
                                                   "median_plain": float(np.median(psat))},
-            "model": os.path.relpath(args.model, ROOT)}
+            "model": os.path.relpath(args.model, ROOT),
+            "tuning": tuning}
 
 
 def main():
@@ -196,7 +199,8 @@ def main():
     cases = wl.cases
     T.init_slots(1, 1, args.queue_mb << 20, 1 << args.dedup_log2)
     from tdp1375_amd import autotune
-    tuning = (autotune.apply(args.tuning, wl.name) if args.tuning else None) or dict(autotune.DEFAULT)
+    tuning = (autotune.apply(args.tuning, autotune.cache_key(wl.name, args.ordering)) if args.tuning else None) \
+        or dict(autotune.DEFAULT)
     pis = [c.pi for c in cases]
     spis = [c.spi for c in cases]
     e0s = [c.even_cycles for c in cases]

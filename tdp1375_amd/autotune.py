@@ -38,6 +38,16 @@ GRID = {
     "batch": [4, 8],
     "max_active": [512, 1024, 2048],
 }
+# The learned ordering settles most configurations within a few expansions
+# and leaves a tail of hard ones, each expansion costing more (the scoring):
+# more frequent and larger hand-offs and larger launches are worth trying.
+GRID_LEARNED = {
+    "waves_per_cu": [0, 8],
+    "share_interval": [2, 4, 8, 16],
+    "share_max": [8, 16, 32],
+    "batch": [4, 8],
+    "max_active": [1024, 2048, 4096],
+}
 DEFAULT_CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "tuning.json")
 
 
@@ -53,9 +63,12 @@ def reset() -> None:
 
 
 def autotune(slot: int, pis, spis, e0s, min_rate, max_depth, grid: Optional[Dict[str, List[int]]] = None,
-             start: Optional[Dict[str, int]] = None, repeats: int = 1, flags: int = 0, log=None) -> dict:
-    """Coordinate descent over grid from start; returns {"best": knobs,
-    "kernel_ms": t, "trials": [...]} and leaves the best tuning set."""
+             start: Optional[Dict[str, int]] = None, repeats: int = 1, flags: int = 0, log=None,
+             passes: int = 1) -> dict:
+    """Coordinate descent over grid from start (up to `passes` sweeps over
+    the knobs, stopping after a sweep that changes nothing); returns
+    {"best": knobs, "kernel_ms": t, "trials": [...]} and leaves the best
+    tuning set."""
     grid = grid or GRID
     best = dict(start or DEFAULT)
     trials = []
@@ -83,16 +96,21 @@ def autotune(slot: int, pis, spis, e0s, min_rate, max_depth, grid: Optional[Dict
     first = G.search_batch(slot, pis, spis, e0s, min_rate, max_depth, 0, flags=flags)
     reference = first.found.copy()
     best_t = run(best)
-    for knob in KNOBS:
-        tried = best[knob]          # already measured as part of best
-        for v in grid.get(knob, []):
-            if v == tried:
-                continue
-            k = dict(best)
-            k[knob] = v
-            t = run(k)
-            if t < best_t:
-                best, best_t = k, t
+    measured = {tuple(sorted(best.items()))}
+    for _ in range(max(1, passes)):
+        before = dict(best)
+        for knob in KNOBS:
+            for v in grid.get(knob, []):
+                k = dict(best)
+                k[knob] = v
+                if tuple(sorted(k.items())) in measured:
+                    continue
+                measured.add(tuple(sorted(k.items())))
+                t = run(k)
+                if t < best_t:
+                    best, best_t = k, t
+        if best == before:
+            break
     set_knobs(best)
     return {"best": best, "kernel_ms": best_t, "trials": trials}
 
@@ -107,6 +125,12 @@ def save(path: str, key: str, result: dict, note: str = "") -> None:
     with open(tmp, "w") as fh:
         json.dump(data, fh, indent=1)
     os.replace(tmp, path)
+
+
+def cache_key(workload: str, ordering: str = "plain") -> str:
+    """Cache entry of a workload under an ordering (the plain one keeps the
+    workload's own name)."""
+    return workload if ordering == "plain" else f"{workload}/{ordering}"
 
 
 def apply(path: str, key: str) -> Optional[Dict[str, int]]:
@@ -129,22 +153,34 @@ def main(argv=None):
     ap.add_argument("--queue-mb", type=int, default=32768)
     ap.add_argument("--dedup-log2", type=int, default=27)
     ap.add_argument("--repeats", type=int, default=1)
+    ap.add_argument("--passes", type=int, default=1, help="coordinate-descent sweeps over the knobs")
+    ap.add_argument("--ordering", choices=("plain", "learned"), default="plain")
+    ap.add_argument("--model", default=os.path.join(os.path.dirname(os.path.abspath(__file__)), "data",
+                                                    "ordering_mlp_d11.npz"))
     args = ap.parse_args(argv)
 
     def log(msg):
         print(f"[{time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
 
     wl = W.workload_odd_cycle_mix_d11(args.cases)
+    flags, grid = 0, GRID
+    if args.ordering == "learned":
+        from . import train_ordering
+        train_ordering.load(args.model)
+        flags, grid = _lib.FLAG_LEARNED, GRID_LEARNED
     G.init_slots(1, 1, args.queue_mb << 20, 1 << args.dedup_log2)
     try:
         res = autotune(0, [c.pi for c in wl.cases], [c.spi for c in wl.cases], [c.even_cycles for c in wl.cases],
-                       wl.min_rate, wl.max_depth, repeats=args.repeats, log=log)
+                       wl.min_rate, wl.max_depth, grid=grid, repeats=args.repeats, flags=flags, log=log,
+                       passes=args.passes)
     finally:
         reset()
         G.destroy_slots()
-    save(args.out, wl.name, res, note=f"{args.cases} configurations, queue {args.queue_mb} MiB, "
-                                      f"dedup 2^{args.dedup_log2}, coordinate descent from the library defaults")
-    print(json.dumps({"workload": wl.name, "best": res["best"], "kernel_ms": res["kernel_ms"],
+    key = cache_key(wl.name, args.ordering)
+    save(args.out, key, res, note=f"{args.cases} configurations, {args.ordering} ordering, queue {args.queue_mb} MiB, "
+                                  f"dedup 2^{args.dedup_log2}, coordinate descent from the library defaults "
+                                  f"({args.passes} pass(es) at most, {args.repeats} run(s) per trial)")
+    print(json.dumps({"workload": key, "best": res["best"], "kernel_ms": res["kernel_ms"],
                       "trials": len(res["trials"])}), flush=True)
 
 

@@ -60,6 +60,33 @@ def test_coordinate_descent_finds_the_minimum_and_caches_it(monkeypatch, tmp_pat
     assert json.load(open(path))["w"]["best"] == res["best"]
 
 
+class Coupled(Fake):
+    """share_interval's best value depends on max_active, a knob swept after
+    it: one pass settles share_interval too early."""
+
+    def time(self):
+        k = self.knobs
+        want = 32 if k["max_active"] == 2048 else 8
+        return 100.0 + abs(k["share_interval"] - want) + (0 if k["max_active"] == 2048 else 50)
+
+
+def test_a_second_pass_revisits_knobs_after_a_later_one_moved(monkeypatch):
+    Coupled(monkeypatch)
+    one = A.autotune(0, [[0]] * 3, [[0]] * 3, [0] * 3, 1.0, 4, passes=1)
+    assert one["best"]["max_active"] == 2048 and one["best"]["share_interval"] == 8
+    two = A.autotune(0, [[0]] * 3, [[0]] * 3, [0] * 3, 1.0, 4, passes=3)
+    assert two["best"]["max_active"] == 2048 and two["best"]["share_interval"] == 32
+    assert two["kernel_ms"] == 100.0
+    # no setting is measured twice
+    seen = [tuple(sorted(t["knobs"].items())) for t in two["trials"]]
+    assert len(seen) == len(set(seen))
+
+
+def test_cache_keys_per_ordering():
+    assert A.cache_key("w") == "w"
+    assert A.cache_key("w", "learned") == "w/learned"
+
+
 def test_a_trial_that_changes_a_verdict_is_an_error(monkeypatch):
     Fake(monkeypatch, flip={"share_max": 4})
     with pytest.raises(RuntimeError, match="verdicts changed"):

@@ -5,10 +5,10 @@
   1.375, D = 9, dedup 2^24, 3 GiB of queue memory, 2 slots): the
   deterministic mode reproduces the oracle's verdict, sequence and every
   counter of every configuration; the work-sharing mode its verdicts;
-* the 4,096-configuration depth-11 batch exactly as bench.py runs it (tuned,
-  work-sharing, dedup 2^27, 32 GiB of queue memory), with the plain and with
-  the learned ordering: every verdict the oracle decided within its expansion
-  budget is equal.
+* the 4,096-configuration depth-11 batch exactly as bench.py runs it (each
+  ordering with its own cached tuning, work-sharing, dedup 2^27, 32 GiB of
+  queue memory), with the plain and with the learned ordering: every verdict
+  the oracle decided within its expansion budget is equal.
 
 Every returned sequence is replayed to an accepted state.
 """
@@ -76,11 +76,14 @@ def _depth11_verdicts():
 
 
 def run_depth11_bench_path(flags=0):
-    """The bench's step: the whole batch, tuned, work-sharing mode."""
+    """The bench's step: the whole batch, tuned for its ordering,
+    work-sharing mode."""
+    from tdp1375_amd import _lib
     wl = W.workload_odd_cycle_mix_d11(4096)
     T.init_slots(1, 1, 32768 << 20, 1 << 27)
     try:
-        autotune.apply(autotune.DEFAULT_CACHE, wl.name)
+        autotune.apply(autotune.DEFAULT_CACHE,
+                       autotune.cache_key(wl.name, "learned" if flags & _lib.FLAG_LEARNED else "plain"))
         got = T.search_batch(0, [c.pi for c in wl.cases], [c.spi for c in wl.cases],
                              [c.even_cycles for c in wl.cases], wl.min_rate, wl.max_depth, 0, flags=flags)
     finally:
