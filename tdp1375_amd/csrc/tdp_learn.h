@@ -95,15 +95,20 @@ __device__ __forceinline__ void mlp_tile_logits(const MlpWeights &M, const uint3
 #pragma unroll
     for (int ks = 0; ks < 4; ++ks) a[ks] = feature_fragment(fw, ks, lane);
     float y[4] = {0.f, 0.f, 0.f, 0.f};
+    // A hidden block's four k-step weight fragments, bias and output weight
+    // are loaded together before its products: one L2 round trip per block,
+    // not one per fragment waited for before its product.
+    const bf16x8_t *wrow = reinterpret_cast<const bf16x8_t *>(M.w1t + (size_t)(lane & 15) * kIn + 8 * (lane >> 4));
+    constexpr int kRowStep = 16 * kIn / 8;   // bf16x8 units from one hidden block to the next
+#pragma unroll 1
     for (int nt = 0; nt < kHidden / 16; ++nt) {
-        const int col = nt * 16 + (lane & 15);
+        bf16x8_t b[4];
+#pragma unroll
+        for (int ks = 0; ks < 4; ++ks) b[ks] = wrow[nt * kRowStep + 4 * ks];
+        const float bias = M.b1[nt * 16 + (lane & 15)], w = (float)M.w2[nt * 16 + (lane & 15)];
         f32x4_t c = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int ks = 0; ks < 4; ++ks) {
-            const bf16x8_t b = *reinterpret_cast<const bf16x8_t *>(M.w1t + (size_t)col * kIn + 32 * ks + 8 * (lane >> 4));
-            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[ks], b, c, 0, 0, 0);
-        }
-        const float bias = M.b1[col], w = (float)M.w2[col];
+        for (int ks = 0; ks < 4; ++ks) c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[ks], b[ks], c, 0, 0, 0);
 #pragma unroll
         for (int i = 0; i < 4; ++i) y[i] += fmaxf(c[i] + bias, 0.f) * w;
     }

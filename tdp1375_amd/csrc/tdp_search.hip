@@ -743,63 +743,104 @@ __device__ __forceinline__ void push_staged(const SearchParams &P, WaveState &W,
     push_child<kServe>(P, W, L, nd, n, sw, lex_index(n, T, i, j, k), i, j, k, (int)((st >> 18) & 63u), m);
 }
 
+// Ascending bitonic sort of 128 (key, payload) pairs held two per lane
+// (element e = slot * 64 + lane); element e holds rank e afterwards.  A
+// compare-exchange swaps only on a strict inequality, which both partners
+// see alike, so equal keys keep a consistent order.
+__device__ __forceinline__ void sort128(float (&key)[2], uint32_t (&val)[2], int lane) {
+#pragma unroll
+    for (int k = 2; k <= 2 * kWave; k <<= 1) {
+#pragma unroll
+        for (int d = k >> 1; d > 0; d >>= 1) {
+            if (d == kWave) {
+                // the lane's two slots, ascending
+                const bool sw = key[1] < key[0];
+                const float k0 = sw ? key[1] : key[0], k1 = sw ? key[0] : key[1];
+                const uint32_t v0 = sw ? val[1] : val[0], v1 = sw ? val[0] : val[1];
+                key[0] = k0; key[1] = k1;
+                val[0] = v0; val[1] = v1;
+            } else {
+#pragma unroll
+                for (int s = 0; s < 2; ++s) {
+                    const int e = s * kWave + lane;
+                    const float pk = __shfl_xor(key[s], d);
+                    const uint32_t pv = (uint32_t)__shfl_xor((int)val[s], d);
+                    const bool keep_min = ((e & d) == 0) == ((e & k) == 0);
+                    const bool sw = keep_min ? (pk < key[s]) : (pk > key[s]);
+                    key[s] = sw ? pk : key[s];
+                    val[s] = sw ? pv : val[s];
+                }
+            }
+        }
+    }
+}
+
 // Learned ordering over a whole batch.  The admitted children of one chunk of
 // the stage (lane l: score, stage entry, when adm) join the children held back
-// so far (L.carry_*); all but the `keep` best of them are pushed in ascending
-// score (ties: chunk before held back, then lane), and the best `keep` are
-// held back again.  keep = kWave between chunks and flushes of a batch, 0 at
-// its end, so the best kWave children of the batch end on top of the stack
-// in score order wherever the stage was cut into chunks and flushes.
+// so far (L.carry_*, ascending); one sort of the 128 candidates ranks them,
+// all but the `keep` best are pushed in ascending score (two per store when a
+// node fits in half a wave), and the best `keep` are held back again.
+// keep = kWave between chunks and flushes of a batch, 0 at its end, so the
+// best kWave children of the batch end on top of the stack in score order
+// wherever the stage was cut into chunks and flushes.
 template <bool kServe>
 __device__ __forceinline__ void learned_push(const SearchParams &P, WaveState &W, WaveLds &L, int n, int sw, int T,
                                              bool adm, float score, uint32_t st, int keep) {
     const int nc = (int)L.ncarry;
-    bool va = adm;
-    bool vb = W.lane < nc;
-    const float sb = vb ? L.carry_score[W.lane] : 0.f;
-    const uint32_t stb = vb ? L.carry_st[W.lane] : 0u;
-    int total = __popcll(__ballot(va)) + nc;
-    while (total > keep && !W.stop) {
-        // the lowest (score, id) of the two candidates of every lane; ids
-        // 0..63 chunk, 64..127 held back, 128 none
-        float key = va ? score : 3.0e38f;
-        int id = va ? W.lane : 128;
-        if (vb && (sb < key || (sb == key && kWave + W.lane < id))) {
-            key = sb;
-            id = kWave + W.lane;
+    const int total = __popcll(__ballot(adm)) + nc;
+    if (total == 0) return;
+    // valid keys within +-2e38 (a NaN score ranks lowest), empty entries 3e38,
+    // so ranks 0 .. total-1 are the candidates
+    float key[2];
+    uint32_t val[2];
+    key[0] = adm ? fminf(fmaxf(score, -2.0e38f), 2.0e38f) : 3.0e38f;
+    val[0] = st;
+    const bool vb = W.lane < nc;
+    key[1] = vb ? L.carry_score[W.lane] : 3.0e38f;
+    val[1] = vb ? L.carry_st[W.lane] : 0u;
+    sort128(key, val, W.lane);
+    const int losers = total > keep ? total - keep : 0;
+    const bool pairs = (int)P.node_words <= kWave / 2;
+    for (int r = 0; r < losers && !W.stop;) {
+        const uint32_t st0 = (uint32_t)__builtin_amdgcn_readlane((int)(r >= kWave ? val[1] : val[0]), r & (kWave - 1));
+        const bool two = pairs && r + 1 < losers && W.hi - W.lo + 2 <= P.stack_cap;
+        if (two) {
+            const int r1 = r + 1;
+            const uint32_t st1 =
+                (uint32_t)__builtin_amdgcn_readlane((int)(r1 >= kWave ? val[1] : val[0]), r1 & (kWave - 1));
+            const bool upper = W.lane >= kWave / 2;
+            const uint32_t cst = upper ? st1 : st0;
+            const int i = (int)(cst & 63u), j = (int)((cst >> 6) & 63u), k = (int)((cst >> 12) & 63u);
+            const uint32_t *nd = L.node[cst >> 24];
+            const uint8_t *cspi = reinterpret_cast<const uint8_t *>(nd + kNodeHeaderWords);
+            const uint8_t *cpi = cspi + 4 * sw;
+            ChildMove cm;
+            cm.a = cpi[i]; cm.b = cpi[j]; cm.c = cpi[k];
+            cm.na = cspi[cm.c]; cm.nb = cspi[cm.a]; cm.nc = cspi[cm.b];
+            const int wl = W.lane & (kWave / 2 - 1);
+            const uint32_t w = child_word(nd, wl, n, sw, lex_index(n, T, i, j, k), i, j, k,
+                                          (int)((cst >> 18) & 63u), cm);
+            uint32_t *dst = stack_slot(P, W, W.hi + (upper ? 1u : 0u));
+            if (wl < (int)P.node_words) dst[wl] = w;
+            W.hi += 2;
+            lds_add(&L.traffic[1], 2);
+            lds_add(&L.live, 2ULL);
+            r += 2;
+        } else {
+            push_staged<kServe>(P, W, L, n, sw, T, st0);
+            r += 1;
         }
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) {
-            const float ok = __shfl_xor(key, off);
-            const int oi = __shfl_xor(id, off);
-            if (ok < key || (ok == key && oi < id)) {
-                key = ok;
-                id = oi;
-            }
-        }
-        id = uniform(id);
-        const uint32_t from = id < kWave ? st : stb;
-        const uint32_t pst = (uint32_t)__builtin_amdgcn_readlane((int)from, id & (kWave - 1));
-        if (id < kWave) va = va && W.lane != id;
-        else vb = vb && W.lane != id - kWave;
-        --total;
-        push_staged<kServe>(P, W, L, n, sw, T, pst);
     }
     wave_sync();   // every lane has read the held-back list before it is rewritten
-    const unsigned long long ma = __ballot(va), mb = __ballot(vb);
-    const int ca = __popcll(ma);
-    const unsigned long long below = (W.lane == 0) ? 0ULL : (~0ULL >> (64 - W.lane));
-    if (va) {
-        const int q = __popcll(ma & below);
-        L.carry_score[q] = score;
-        L.carry_st[q] = st;
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+        const int r = s * kWave + W.lane;
+        if (r >= losers && r < total) {
+            L.carry_score[r - losers] = key[s];
+            L.carry_st[r - losers] = val[s];
+        }
     }
-    if (vb) {
-        const int q = ca + __popcll(mb & below);
-        L.carry_score[q] = sb;
-        L.carry_st[q] = stb;
-    }
-    if (W.lane == 0) L.ncarry = (uint32_t)(ca + __popcll(mb));
+    if (W.lane == 0) L.ncarry = (uint32_t)(total - losers);
     wave_sync();
 }
 
