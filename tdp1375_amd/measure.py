@@ -18,6 +18,16 @@ filter), the wall time of the call and the verdict or row counts:
 
 The configuration descriptors are uploaded by the first call and stay
 resident for the repeats (TDP_FLAG_RESIDENT), as in bench.py.
+
+    python -m tdp1375_amd.measure --learned-ablation
+
+instead runs the depth-11 batch (4,096 configurations, bench path, each
+ordering with its cached tuning) with the plain kernel and with the learned
+kernel under different networks: the committed model, an all-zero network
+(every child scored alike), two random initialisations and the committed
+model with its output negated.  One JSON line per variant: kernel time of two
+calls, expansions to verdict over the satisfiable configurations and their
+median; verdicts must equal the plain search's.
 """
 from __future__ import annotations
 
@@ -81,12 +91,71 @@ def _filter(n: int, skip: bool, repeats: int) -> dict:
             "ranks_per_s": ranks / (1e-3 * best_kernel) if best_kernel else None}
 
 
+def learned_ablation() -> None:
+    import os
+
+    import numpy as np
+
+    from . import autotune, learned as LN, train_ordering
+    wl = W.workload_odd_cycle_mix_d11(4096)
+    pis, spis, e0s = [c.pi for c in wl.cases], [c.spi for c in wl.cases], [c.even_cycles for c in wl.cases]
+    model = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "ordering_mlp_d11.npz")
+    G.init_slots(1, 1, 32768 << 20, 1 << 27)
+    try:
+        def run(flags, ordering):
+            autotune.apply(autotune.DEFAULT_CACHE, autotune.cache_key(wl.name, ordering))
+            try:
+                out = []
+                for rep in range(2):
+                    res = G.search_batch(0, pis, spis, e0s, wl.min_rate, wl.max_depth, 0,
+                                         flags=flags | (_lib.FLAG_RESIDENT if rep else 0))
+                    out.append((res, G.last_launch_stats(0)["kernel_ms"]))
+            finally:
+                autotune.reset()
+            return out
+
+        plain = run(0, "plain")
+        verdicts = plain[-1][0].found.copy()
+
+        def report(name, runs):
+            res = runs[-1][0]
+            if not np.array_equal(res.found, verdicts):
+                raise RuntimeError(f"{name}: verdicts differ from the plain search's")
+            e = res.counters["expansions"][verdicts].astype(np.float64)
+            print(json.dumps({"variant": name, "kernel_ms": [round(k, 1) for _, k in runs],
+                              "satisfiable": int(verdicts.sum()), "expansions_to_verdict": float(e.sum()),
+                              "median": float(np.median(e))}), flush=True)
+
+        report("plain kernel", plain)
+        with np.load(model, allow_pickle=False) as z:
+            w1, b1, w2, b2 = z["w1"], z["b1"], z["w2"], float(z["b2"])
+        try:
+            LN.set_weights(w1, b1, w2, b2)
+            report("learned kernel, committed model", run(_lib.FLAG_LEARNED, "learned"))
+            LN.set_weights(np.zeros_like(w1), np.zeros_like(b1), np.zeros_like(w2), 0.0)
+            report("learned kernel, all-zero network", run(_lib.FLAG_LEARNED, "learned"))
+            for seed in (1, 2):
+                LN.init(seed)
+                report(f"learned kernel, random network (seed {seed})", run(_lib.FLAG_LEARNED, "learned"))
+            LN.set_weights(w1, b1, -w2, -b2)
+            report("learned kernel, committed model negated", run(_lib.FLAG_LEARNED, "learned"))
+        finally:
+            train_ordering.load(model)
+    finally:
+        G.destroy_slots()
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--repeats", type=int, default=3)
+    ap.add_argument("--learned-ablation", action="store_true",
+                    help="the depth-11 batch under the plain kernel and under the learned kernel with several networks")
     args = ap.parse_args()
     if _lib.load().tdp_device_count() < 1:
         raise SystemExit("no GPU visible")
+    if args.learned_ablation:
+        learned_ablation()
+        return
     for wl in (W.workload_random_n12(), W.workload_four_3cycles_n24(True), W.workload_four_3cycles_n24(False)):
         print(json.dumps(_search(wl, args.repeats)), flush=True)
     for n in (12, 11):

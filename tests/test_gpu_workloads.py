@@ -75,11 +75,11 @@ def _depth11_verdicts():
         return json.load(fh)
 
 
-def run_depth11_bench_path(flags=0):
-    """The bench's step: the whole batch, tuned for its ordering,
-    work-sharing mode."""
+def run_depth11_bench_path(flags=0, cases=4096):
+    """The bench's step: the whole batch (or its first `cases`
+    configurations), tuned for its ordering, work-sharing mode."""
     from tdp1375_amd import _lib
-    wl = W.workload_odd_cycle_mix_d11(4096)
+    wl = W.workload_odd_cycle_mix_d11(cases)
     T.init_slots(1, 1, 32768 << 20, 1 << 27)
     try:
         autotune.apply(autotune.DEFAULT_CACHE,
@@ -120,3 +120,38 @@ def test_depth11_batch_learned_ordering_verdicts(gpu_lib):
     wl, got = run_depth11_bench_path(flags=_lib.FLAG_LEARNED)
     decided = check_depth11(wl, got)
     assert decided >= 3900
+
+
+def test_trained_network_gains_over_null_network_and_plain(gpu_lib):
+    """What the trained network gains, on the first 1,024 configurations of
+    the depth-11 batch (bench path): the learned kernel with the committed
+    model against the same kernel with an all-zero network (the same
+    hold-back and push machinery, every child scored alike) and against the
+    plain ordering.  Verdicts are equal in all three; expansions to verdict
+    over the satisfiable configurations must be at least 3x fewer with the
+    trained network than with either (measured on the whole batch: 3.2e7
+    trained, 9.3e8 plain, 2.7e9 zero network)."""
+    from tdp1375_amd import _lib, learned as LN, train_ordering
+    model = os.path.join(os.path.dirname(T.__file__), "data", "ordering_mlp_d11.npz")
+    _, plain = run_depth11_bench_path(cases=1024)
+    with np.load(model, allow_pickle=False) as z:
+        w1, b1, w2 = z["w1"], z["b1"], z["w2"]
+    LN.set_weights(np.zeros_like(w1), np.zeros_like(b1), np.zeros_like(w2), 0.0)
+    try:
+        _, null = run_depth11_bench_path(flags=_lib.FLAG_LEARNED, cases=1024)
+    finally:
+        train_ordering.load(model)
+    wl, trained = run_depth11_bench_path(flags=_lib.FLAG_LEARNED, cases=1024)
+    assert np.array_equal(plain.found, null.found) and np.array_equal(plain.found, trained.found)
+    verdicts = _depth11_verdicts()["verdicts"][:1024]
+    decided = [c for c, v in enumerate(verdicts) if v != "?"]
+    assert np.array_equal(trained.found[decided], np.array([verdicts[c] == "1" for c in decided]))
+    sat = plain.found
+    exp = {name: float(r.counters["expansions"][sat].sum()) for name, r in
+           (("plain", plain), ("zero network", null), ("trained", trained))}
+    print(f"satisfiable {int(sat.sum())} of {len(sat)}: expansions to verdict {exp}", flush=True)
+    assert exp["plain"] >= 3.0 * exp["trained"]
+    assert exp["zero network"] >= 3.0 * exp["trained"]
+    for c, case in enumerate(wl.cases):
+        if trained.found[c]:
+            assert_witness(case.pi, case.spi, case.even_cycles, wl.min_rate, wl.max_depth, trained.moves[c])
