@@ -10,8 +10,9 @@ and stay resident in HBM (TDP_FLAG_RESIDENT).
 Prints one JSON line: states expanded per second (value), cases per hour,
 the search kernel's roofline figure (algorithmic queue + dedup bytes of the
 launch, from the kernel's own counters, over its HIP-event duration on the
-slot's stream), the measured HBM bytes per launch when the rocprofv3 counter
-summary of this workload is present under profiles/, the kernel's vector-ALU
+slot's stream), the measured HBM bytes per launch (the committed rocprofv3 counter
+summary's bytes per expansion times this run's expansions per launch) when
+that summary is present under profiles/, the kernel's vector-ALU
 issue rate against the chip's (VALU instructions per expansion from the
 committed SQ-counter summary times the live expansion rate), the learned
 ordering timed on the same resident batch (``learned``: states/s, cases/h
@@ -20,8 +21,9 @@ search's), and the CPU baseline (the C restatement of the reference search,
 on a bounded sample of the same workload, over the host's cores, whose
 decided verdicts must equal the GPU's).
 
-``--copy-calibration MB`` instead runs a plain device copy of MB MiB (the
-known-size kernel that scales the FETCH_SIZE / WRITE_SIZE counters).
+``--copy-calibration MB`` instead runs a plain device copy of MB MiB, and
+``--gather-calibration MB`` a gather of MB MiB of random 64-byte rows (the
+known-size kernels that scale the read and write request counters).
 """
 from __future__ import annotations
 
@@ -60,14 +62,14 @@ def algorithmic_bytes(stats: dict, counters: dict) -> int:
 
 
 def measured_traffic(workload: str):
-    """HBM bytes per search_kernel dispatch from the committed rocprofv3
-    counter summary of this workload (FETCH_SIZE + WRITE_SIZE, scaled by the
-    copy calibration), or None when it is absent."""
+    """HBM bytes (read + written) per expansion of search_kernel from the
+    committed rocprofv3 counter summary of this workload (the L2's memory-side
+    requests counted per request size), or None when it is absent."""
     path = os.path.join(ROOT, "profiles", f"search_kernel_{workload}_hbm_bytes.json")
     if not os.path.exists(path):
         return None
     with open(path) as fh:
-        return json.load(fh).get("hbm_bytes_per_dispatch")
+        return json.load(fh).get("per_expansion", {}).get("bytes")
 
 
 def sq_profile(workload: str):
@@ -89,6 +91,25 @@ def copy_calibration(mb: int) -> None:
         dst.copy_(src)
     torch.cuda.synchronize()
     print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4, "dispatches": 3}))
+
+
+def gather_calibration(mb: int) -> None:
+    """Random 64-byte rows (16 float32, 64-byte aligned) gathered from an
+    8 GiB table, MB MiB of rows per dispatch: the access pattern of the dedup
+    table's tag and key lines, for scaling the read-request counters."""
+    import torch
+    rows_table = (8 << 30) // 64
+    rows = (mb << 20) // 64
+    table = torch.ones(rows_table, 16, dtype=torch.float32, device="cuda:0")
+    g = torch.Generator(device="cuda:0")
+    g.manual_seed(7)
+    idx = torch.randint(0, rows_table, (rows,), device="cuda:0", generator=g)
+    out = torch.empty(rows, 16, dtype=torch.float32, device="cuda:0")
+    for _ in range(3):
+        torch.index_select(table, 0, idx, out=out)
+    torch.cuda.synchronize()
+    print(json.dumps({"gather_rows": rows, "row_bytes": 64, "index_bytes": rows * 8,
+                      "bytes_written": rows * 64, "dispatches": 3}))
 
 
 def cpu_baseline(wl, sample_cases: int, cap: int, threads: int, gpu_found):
@@ -167,6 +188,7 @@ def main():
     ap.add_argument("--learned-steps", type=int, default=2,
                     help="timed steps of the learned ordering on the same batch (0: skip)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB")
+    ap.add_argument("--gather-calibration", type=int, default=0, metavar="MB")
     ap.add_argument("--graph", action="store_true",
                     help="run each launch round as the slot's HIP graph (TDP_FLAG_GRAPH)")
     ap.add_argument("--ordering", choices=("plain", "learned"), default="plain",
@@ -185,6 +207,9 @@ def main():
 
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
+        return
+    if args.gather_calibration:
+        gather_calibration(args.gather_calibration)
         return
 
     import numpy as np
@@ -260,6 +285,8 @@ def main():
     avg_kernel_s = 1e-3 * sum(kernel_ms) / len(kernel_ms) / dispatches
     bytes_per_dispatch = sum(traffic) / len(traffic) / dispatches
     achieved = bytes_per_dispatch / avg_kernel_s / 1e9
+    exps_per_dispatch = exps / args.steps / dispatches
+    measured_per_exp = measured_traffic(wl.name) if args.ordering == "plain" else None
     cases_per_hour = len(cases) * args.steps / wall * 3600.0
     # the graph's events bracket the whole round (reset kernel, search kernel,
     # control copy), so in graph mode the time is a round's, not the kernel's
@@ -293,7 +320,9 @@ def main():
         "cases_per_hour": cases_per_hour,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": measured_traffic(wl.name) if args.ordering == "plain" else None,
+                     "traffic": measured_per_exp * exps_per_dispatch if measured_per_exp else None,
+                     "traffic_bytes_per_expansion": measured_per_exp,
+                     "algorithmic_bytes_per_expansion": bytes_per_dispatch / exps_per_dispatch,
                      "kernel": kernel_label, "dispatches_per_step": dispatches,
                      "avg_kernel_ms": 1e3 * avg_kernel_s,
                      "algorithmic_bytes_per_launch": bytes_per_dispatch,

@@ -94,6 +94,11 @@ __device__ __forceinline__ void wave_sync() {
 }
 
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+// Fields of the wave's LDS descriptor copy as wave-uniform (scalar) values.
+__device__ __forceinline__ int cfg_int(const uint32_t &f) { return uniform((int)f); }
+__device__ __forceinline__ uint64_t cfg_bits(const uint64_t &f) {
+    return ((uint64_t)(uint32_t)uniform((int)(f >> 32)) << 32) | (uint32_t)uniform((int)(uint32_t)f);
+}
 
 // Inclusive prefix sum / running maximum over the 64 lanes with DPP row
 // shifts and row broadcasts (a few VALU cycles each, instead of LDS-latency
@@ -1086,8 +1091,8 @@ __device__ __forceinline__ void flush_stage(const SearchParams &P, WaveState &W,
             const int e = (int)((nd[1] >> 8) & 0xFF);
             uint32_t f[kFeatWords];
             const uint32_t *pinfo = L.posinfo[st >> 24];   // the child's parent
-            move_features(pinfo[i], pinfo[j], pinfo[k], i, j, k, n, d, (int)C.max_depth,
-                          (int)C.e_start, ce, ce - e, f);
+            move_features(pinfo[i], pinfo[j], pinfo[k], i, j, k, n, d, cfg_int(C.max_depth),
+                          cfg_int(C.e_start), ce, ce - e, f);
             const unsigned long long below = (W.lane == 0) ? 0ULL : (pm & (~0ULL >> (64 - W.lane)));
             const int ci = __popcll(below);
             const int cnt = __popcll(pm);
@@ -1414,10 +1419,10 @@ __device__ __forceinline__ int cycle_local_children(WaveState &W, WaveLds &L, co
 template <bool kLearned, bool kServe>
 __device__ __forceinline__ int expand_stage(const SearchParams &P, WaveState &W, WaveLds &L, const ConfigDesc &C,
                                             const uint32_t *nd, int b, bool root_mode, int &scount) {
-    const int n = (int)C.n;
+    const int n = cfg_int(C.n);
     const int sw = (int)P.spi_words;
-    const int T = (int)C.triple_count;
-    const int D = (int)C.max_depth;
+    const int T = cfg_int(C.triple_count);
+    const int D = cfg_int(C.max_depth);
     const int d = (int)(nd[1] & 0xFF);
     const int e = (int)((nd[1] >> 8) & 0xFF);
     // the learned ordering scores children at the flush from their parent's
@@ -1433,8 +1438,8 @@ __device__ __forceinline__ int expand_stage(const SearchParams &P, WaveState &W,
     // by -2, 0 or +2, so each expansion reduces to two 3-bit masks over
     // delta in {-2, 0, +2}: which deltas make a goal child, which make an
     // admissible one.
-    const uint64_t acc = C.accept[d + 1];
-    const uint64_t rch = (d + 1 < D) ? C.reach[d + 1] : 0ULL;
+    const uint64_t acc = cfg_bits(C.accept[d + 1]);
+    const uint64_t rch = (d + 1 < D) ? cfg_bits(C.reach[d + 1]) : 0ULL;
     uint32_t gmask = 0, amask = 0;
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
@@ -1521,10 +1526,20 @@ __device__ __forceinline__ int expand_stage(const SearchParams &P, WaveState &W,
     return goal_t;
 }
 
-__device__ __forceinline__ void switch_config(const SearchParams &P, WaveState &W, WaveLds &L, int cfg) {
+// A batch launch's descriptors do not change while it runs: the wave keeps
+// its current configuration's descriptor in LDS, so the per-node predicate
+// tables and sizes are LDS reads rather than vector loads that would also
+// wait for the wave's outstanding stores.
+__device__ __forceinline__ void switch_config(const SearchParams &P, WaveState &W, WaveLds &L, ConfigDesc &copy,
+                                              int cfg) {
     if (cfg != W.cur_cfg) {
         flush_counters<false>(P, W, L);
         W.cur_cfg = cfg;
+        const uint32_t *src = reinterpret_cast<const uint32_t *>(P.cfgs + cfg);
+        uint32_t *dst = reinterpret_cast<uint32_t *>(&copy);
+        constexpr int kWords = (int)(sizeof(ConfigDesc) / 4);
+        for (int w = W.lane; w < kWords; w += kWave) dst[w] = src[w];
+        wave_sync();
     }
 }
 
@@ -1683,8 +1698,9 @@ __device__ __forceinline__ void search_body(const SearchParams &P) {
     __shared__ WaveLds lds[kWavesPerBlock];
     __shared__ uint16_t colex[kColexPairs];
     __shared__ uint32_t colex3[kCycleLocalMax];
-    // serving: each wave's copy of the descriptor of its current request
-    __shared__ ConfigDesc cfg_copy[kServe ? kWavesPerBlock : 1];
+    // each wave's copy of the descriptor of its current configuration
+    // (serving: of its current request)
+    __shared__ ConfigDesc cfg_copy[kWavesPerBlock];
     fill_colex(colex, colex3);
     const int wib = threadIdx.x >> 6;
     WaveLds &L = lds[wib];
@@ -1799,9 +1815,9 @@ __device__ __forceinline__ void search_body(const SearchParams &P) {
                     continue;
                 }
                 // start node of configuration c, expanded in root mode
-                switch_config(P, W, L, c);
+                switch_config(P, W, L, cfg_copy[wib], c);
             }
-            const ConfigDesc &C = kServe ? cfg_copy[wib] : P.cfgs[c];
+            const ConfigDesc &C = cfg_copy[wib];
             const int sw = (int)P.spi_words;
             if (W.lane < (int)P.node_words) {
                 uint32_t w = 0;
@@ -1814,7 +1830,7 @@ __device__ __forceinline__ void search_body(const SearchParams &P) {
                 L.node[0][W.lane] = w;
             }
             wave_sync();
-            if (C.triple_count > 0) {
+            if (cfg_int(C.triple_count) > 0) {
                 PHASE_T0();
                 int scount = 0;
                 if (kLearned && W.lane == 0) L.ncarry = 0;
@@ -1822,10 +1838,10 @@ __device__ __forceinline__ void search_body(const SearchParams &P) {
                 expand_stage<kLearned, kServe>(P, W, L, C, L.node[0], 0, true, scount);
                 if (scount && !W.stop) {
                     wave_sync();
-                    flush_stage<kLearned, kServe>(P, W, L, C, scount, (int)C.n, (int)P.spi_words, (int)C.triple_count, true);
+                    flush_stage<kLearned, kServe>(P, W, L, C, scount, cfg_int(C.n), (int)P.spi_words, cfg_int(C.triple_count), true);
                 }
                 if (kLearned && !W.stop)
-                    learned_push<kServe>(P, W, L, (int)C.n, (int)P.spi_words, (int)C.triple_count, false, 0.f, 0u, 0);
+                    learned_push<kServe>(P, W, L, cfg_int(C.n), (int)P.spi_words, cfg_int(C.triple_count), false, 0.f, 0u, 0);
                 PHASE_ADD(L, kPhaseRoot);
             }
             if (kServe && !W.stop) serve_flush(P, W, L);
@@ -1910,7 +1926,7 @@ __device__ __forceinline__ void search_body(const SearchParams &P) {
         // ---- expand the nodes in L.node[0..batch) (one configuration): their
         // children are staged together and deduplicated in one pass
         const int cfg = uniform((int)L.node[0][0]);
-        const ConfigDesc &C = kServe ? cfg_copy[wib] : P.cfgs[cfg];
+        const ConfigDesc &C = cfg_copy[wib];
         {
             PHASE_T0();
             bool skip;
@@ -1918,7 +1934,7 @@ __device__ __forceinline__ void search_body(const SearchParams &P) {
                 skip = serve_switch(P, W, L, cfg_copy[wib], cfg);
                 lds_add(&L.takes, (unsigned long long)batch);   // the nodes just taken
             } else {
-                switch_config(P, W, L, cfg);
+                switch_config(P, W, L, cfg_copy[wib], cfg);
                 lds_add(&L.live, (unsigned long long)(-(long long)batch));   // the nodes just taken
                 skip = config_found(P, cfg);
             }
@@ -1945,14 +1961,14 @@ __device__ __forceinline__ void search_body(const SearchParams &P) {
             const uint32_t *nd = L.node[b];
             const int d = uniform((int)(nd[1] & 0xFF));
             const int e = uniform((int)((nd[1] >> 8) & 0xFF));
-            if ((C.accept[d] >> e) & 1ULL) {
+            if ((cfg_bits(C.accept[d]) >> e) & 1ULL) {
                 report_found<kServe>(P, W, nd, cfg, d, -1);
                 terminal = true;
                 break;
             }
-            if (d >= (int)C.max_depth || !((C.reach[d] >> e) & 1ULL)) continue;
+            if (d >= cfg_int(C.max_depth) || !((cfg_bits(C.reach[d]) >> e) & 1ULL)) continue;
             lds_add(&L.cnt[kCntExpansions], 1);
-            lds_add(&L.cnt[kCntGenerated], C.triple_count);
+            lds_add(&L.cnt[kCntGenerated], (unsigned long long)cfg_int(C.triple_count));
             ++expanded;
             const int goal = expand_stage<kLearned, kServe>(P, W, L, C, nd, b, false, scount);
             if (goal >= 0) {
@@ -1968,10 +1984,10 @@ __device__ __forceinline__ void search_body(const SearchParams &P) {
         if (!terminal && !W.stop) {
             if (scount) {
                 wave_sync();
-                flush_stage<kLearned, kServe>(P, W, L, C, scount, (int)C.n, (int)P.spi_words, (int)C.triple_count, false);
+                flush_stage<kLearned, kServe>(P, W, L, C, scount, cfg_int(C.n), (int)P.spi_words, cfg_int(C.triple_count), false);
             }
             if (kLearned && !W.stop)
-                learned_push<kServe>(P, W, L, (int)C.n, (int)P.spi_words, (int)C.triple_count, false, 0.f, 0u, 0);
+                learned_push<kServe>(P, W, L, cfg_int(C.n), (int)P.spi_words, cfg_int(C.triple_count), false, 0.f, 0u, 0);
             // nodes of the batch not expanded return on top, the top one last
             for (int b = rest - 1; b >= 0 && !W.stop; --b) push_node<kServe>(P, W, L, L.node[b]);
             wave_sync();
