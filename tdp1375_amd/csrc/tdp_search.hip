@@ -54,7 +54,6 @@ struct WaveLds {
     uint32_t memb[64];                       // cycle members in spi order, cycles of length >= 2:
                                              // cycle info | position in pi << 24
     uint32_t ememb[64];                      // the members of the even cycles again, contiguous
-    uint8_t posof[64];                       // position of each symbol in pi
     uint32_t stage[kStage];
     uint8_t rowsrc[64];                      // learned ordering: lane of each compacted admitted child
     float carry_score[kWave];                // learned ordering: the best-scored admitted children of
@@ -347,31 +346,25 @@ __device__ __forceinline__ uint32_t build_posinfo(WaveLds &L, const uint32_t *nd
     const uint8_t *pi = spi + 4 * sw;
     const int s = lane;
     const int nxt = (s < n) ? (int)spi[s] : s;
-    // m: minimum over the 2^r symbols from s on, p: the symbol 2^r steps on.
-    // Once every window has the same minimum as the next one, each holds its
-    // cycle's minimum (the windows cover the cycle): short cycles finish in
-    // fewer than the 6 rounds that lengths up to 64 need.
-    int m = s, p = nxt;
+    // w: the minimum over the window of 2^r symbols from s on (bits 0..7)
+    // and the steps from s to its first occurrence (bits 8..15); p: the
+    // symbol 2^r steps on.  Once every window has the same minimum as the
+    // next one, each holds its cycle's minimum (otherwise some window
+    // without it is followed by one with it), so its steps field is the
+    // distance from s to the cycle's minimum: short cycles finish in fewer
+    // than the 6 rounds that lengths up to 64 need.
+    int w = s, p = nxt;
     for (int r = 0; r < 6; ++r) {
-        const int mo = __shfl(m, p);
-        if (__ballot(mo != m) == 0) break;
+        const int wo = __shfl(w, p);
+        if (__ballot((wo & 0xFF) != (w & 0xFF)) == 0) break;
         const int pp = __shfl(p, p);
-        m = min(m, mo);
+        if ((wo & 0xFF) < (w & 0xFF)) w = wo + (1 << (8 + r));
         p = pp;
     }
-    const int root = m;
-    // f: steps from s to q, q: 2^r steps on or the root (where it stays)
-    int f = (s == root) ? 0 : 1;
-    int q = (s == root) ? s : nxt;
-    for (int r = 0; r < 6; ++r) {
-        if (__ballot(q != root) == 0) break;
-        const int fq = __shfl(f, q);
-        const int qq = __shfl(q, q);
-        f += fq;
-        q = qq;
-    }
+    const int root = w & 0xFF;
     // f is the number of steps to the root, so the root's successor has
     // f = len - 1 (a fixed point is its own successor with f = 0)
+    const int f = w >> 8;
     const int f_next = __shfl(f, nxt);
     const int len = __shfl(f_next + 1, root);
     const int pos = (f == 0) ? 0 : len - f;
@@ -1315,10 +1308,11 @@ __device__ __forceinline__ int cycle_local_children(WaveState &W, WaveLds &L, co
     // bits 24..29, so that one LDS read yields both and sorting three entries
     // sorts them by position
     const uint8_t *pi = reinterpret_cast<const uint8_t *>(nd + kNodeHeaderWords) + 4 * sw;
-    if (s < n) L.posof[pi[s]] = (uint8_t)s;
-    const int my_moff = __shfl(moff, root), my_eoff = __shfl(eoff, root);
-    wave_sync();
-    const uint32_t entry = sinfo | ((uint32_t)L.posof[s < n ? s : 0] << 24);
+    // the position of symbol s in pi: lane s sends s to lane pi[s]
+    const int pos_in_pi = __builtin_amdgcn_ds_permute((s < n ? (int)pi[s] : s) << 2, s);
+    const int offs = __shfl(moff | (eoff << 16), root);
+    const int my_moff = offs & 0xFFFF, my_eoff = offs >> 16;
+    const uint32_t entry = sinfo | ((uint32_t)pos_in_pi << 24);
     if (s < n && len >= 2) L.memb[my_moff + pos] = entry;
     if (s < n && (len & 1) == 0) L.ememb[my_eoff + pos] = entry;
     wave_sync();
