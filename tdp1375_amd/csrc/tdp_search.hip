@@ -884,10 +884,35 @@ __device__ __forceinline__ unsigned long long key_pair(const WaveLds &L, int lan
     return lo | (hi << 32);
 }
 
+// Whether the entry with tag tv may be replaced before a live one: its
+// configuration is found (batch launches), or its slot has moved to another
+// request or its request is done (serving).  f is the loaded found flag or
+// slot state word of the entry's configuration.
+template <bool kServe>
+__device__ __forceinline__ bool entry_done(unsigned long long tv, unsigned long long f) {
+    if (kServe) return (f >> 2) != (tv >> 48) || (f & kSlotDone);
+    return f != 0;
+}
+template <bool kServe>
+__device__ __forceinline__ unsigned long long entry_done_word(const SearchParams &P, unsigned long long tv) {
+    const unsigned long long c = (tv >> 32) & 0xFFFFu;
+    if (kServe) return ld_relaxed(&P.slot_state[c]);
+    return (unsigned long long)__hip_atomic_load(&P.results[c].found, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // Serving (P.serve): the epoch field of an identity is the generation of the
 // request's slot (gen), a free entry is a zero tag, and an entry is evictable
 // first when its slot has moved to another request (other generation) or its
 // request is done (found, cancelled or drained).
+//
+// One round per lookup in the common cases, with the memory accesses of the
+// lanes that compare a stored key and of the lanes that insert issued
+// together: (1) the bucket's tags; (2) the key of the first entry of this
+// identity, or the found flags of a full bucket's configurations; (3) the
+// key compare and the tag re-read, or the claim of a victim entry by CAS;
+// (4) dominated / improved, or the key written and the entry published.  A
+// lane whose stored key of the same identity differs (a 27-bit collision)
+// goes on with sequential rounds over all the candidates.
 template <bool kServe>
 __device__ __forceinline__ int dedup_admit(const SearchParams &P, WaveLds &L, int lane, bool active, uint32_t cfg,
                                            int nw, int depth, uint32_t gen) {
@@ -909,130 +934,214 @@ __device__ __forceinline__ int dedup_admit(const SearchParams &P, WaveLds &L, in
     unsigned long long *tags = P.dd_tag + bucket * kDedupBucket;
     const uint32_t *keys = P.dd_key + bucket * kDedupBucket * 16;
     const int width = (int)P.dd_width;
+    const int nw2 = (nw + 1) >> 1;
     const unsigned long long epoch = gen;
     int res = active ? kDedupPending : kDedupDominated;
     int rounds = 0;
+    bool slow = false;
+    // the victim for this child: a free entry (serving: a zero tag; batch: an
+    // entry of another epoch), else an entry of a configuration already done,
+    // else the deepest entry not shallower than the child; never an entry of
+    // the child's own identity
+    auto free_victim = [&](const unsigned long long (&t)[kDedupBucket]) -> int {
+        int v = -1;
+#pragma unroll
+        for (int q = kDedupBucket - 1; q >= 0; --q)
+            if (q < width && (kServe ? t[q] == 0ULL : (t[q] >> 48) != epoch)) v = q;
+        return v;
+    };
+    auto full_victim = [&](const unsigned long long (&t)[kDedupBucket], const unsigned long long (&f)[kDedupBucket],
+                           uint32_t same_ident, bool &live) -> int {
+        uint32_t done_mask = 0;
+#pragma unroll
+        for (int q = 0; q < kDedupBucket; ++q)
+            if (q < width && (t[q] & kTagReady) && !((same_ident >> q) & 1u) && entry_done<kServe>(t[q], f[q]))
+                done_mask |= 1u << q;
+        if (done_mask) {
+            live = false;
+            return __ffs(done_mask) - 1;
+        }
+        int v = -1, vd = depth - 1;
+#pragma unroll
+        for (int q = 0; q < kDedupBucket; ++q) {
+            const int dq = (int)(t[q] & 15ULL);
+            if (q < width && (t[q] & kTagReady) && !((same_ident >> q) & 1u) && dq > vd) {
+                vd = dq;
+                v = q;
+            }
+        }
+        live = true;
+        return v;
+    };
     while (true) {
         const unsigned long long pm = __ballot(res == kDedupPending);
         if (pm == 0) break;
         lds_add(&L.traffic[2], (unsigned long long)width * (unsigned long long)__popcll(pm));
-        if (res == kDedupPending) {
-            unsigned long long t[kDedupBucket];
+        const bool pend = res == kDedupPending;
+        unsigned long long t[kDedupBucket];
 #pragma unroll
-            for (int q = 0; q < kDedupBucket; ++q) t[q] = q < width ? ld_relaxed(tags + q) : 0ULL;
+        for (int q = 0; q < kDedupBucket; ++q) t[q] = (pend && q < width) ? ld_relaxed(tags + q) : 0ULL;
+        bool retry = false, busy = false;
+        uint32_t cand = 0;   // ready entries with this identity
+#pragma unroll
+        for (int q = 0; q < kDedupBucket; ++q) {
+            const bool same = pend && q < width && (t[q] & ~31ULL) == ident;
+            busy |= same && !(t[q] & kTagReady);
+            cand |= (same && (t[q] & kTagReady)) ? (1u << q) : 0u;
+        }
+        const uint32_t same_ident = cand;   // never evicted for this child
+        const bool fast = pend && !slow, seq = pend && slow;
 
-            bool retry = false, busy = false;
-            uint32_t cand = 0;   // ready entries with this identity
+        // (2) the first candidate's key; a lane to insert picks a free entry
+        // or loads the found flags of the bucket's configurations
+        const bool cmp = fast && cand != 0;
+        const int q1 = cmp ? __ffs(cand) - 1 : 0;
+        bool ins = fast && cand == 0;
+        if (ins && busy) {
+            retry = true;   // the same identity is being written
+            ins = false;
+        }
+        int victim = ins ? free_victim(t) : -1;
+        const bool full = ins && victim < 0;
+        // one register array for both: the key words (comparing lanes) or
+        // the found words of the bucket's entries (lanes with a full bucket)
+        unsigned long long kv[8];
 #pragma unroll
-            for (int q = 0; q < kDedupBucket; ++q) {
-                const bool same = q < width && (t[q] & ~31ULL) == ident;
-                busy |= same && !(t[q] & kTagReady);
-                cand |= (same && (t[q] & kTagReady)) ? (1u << q) : 0u;
+        for (int w = 0; w < 8; ++w) kv[w] = 0ULL;
+        if (cmp) {
+            const unsigned long long *kw = reinterpret_cast<const unsigned long long *>(keys + q1 * 16);
+#pragma unroll
+            for (int w = 0; w < 8; ++w)
+                if (w < nw2) kv[w] = ld_relaxed(kw + w);
+        } else if (full) {
+#pragma unroll
+            for (int q = 0; q < kDedupBucket; ++q)
+                if (q < width && (t[q] & kTagReady)) kv[q] = entry_done_word<kServe>(P, t[q]);
+        }
+
+        // (3) compare and re-read the tag; claim the victim
+        const unsigned long long tq = pick_tag(t, q1);
+        bool eq = false;
+        unsigned long long tq2 = 0;
+        if (cmp) {
+            eq = true;
+#pragma unroll
+            for (int w = 0; w < 8; ++w)
+                if (w < nw2) eq &= kv[w] == key_pair(L, lane, w, nw);
+            count_active(&L.traffic[3], __popcll(__builtin_amdgcn_read_exec()));
+            // The snapshot tag decides only if it still holds after the key
+            // was read: an entry evicted and rewritten meanwhile (by a key of
+            // the same 27-bit identity) changes the tag word, and the lookup
+            // starts over instead of judging this key by the depth of the one
+            // the snapshot described.  What remains is an ABA of the whole
+            // tag word (same identity, same depth, ready again) inside the
+            // window, in which case the entry holds this key at that depth or
+            // a key written after it.
+            if (eq) tq2 = ld_relaxed(tags + q1);
+        }
+        bool live = false;
+        if (full) victim = full_victim(t, kv, same_ident, live);
+        if (ins && victim < 0) res = kDedupUninserted;
+        const bool claim = ins && victim >= 0;
+        bool claimed = false;
+        if (claim) claimed = cas64(tags + victim, pick_tag(t, victim), ident | (unsigned long long)depth);
+
+        // (4) dominated or improved; write the claimed entry's key
+        bool improve = false;
+        if (cmp) {
+            if (!eq) {
+                slow = true;   // another key of this identity: sequential rounds
+                retry = true;
+            } else if (tq2 != tq) {
+                retry = true;
+            } else if ((int)(tq & 15ULL) <= depth) {
+                res = kDedupDominated;
+            } else {
+                improve = true;
             }
-            const uint32_t same_ident = cand;   // never evicted for this child
-            // 1. compare the stored keys of the candidates
+        }
+        bool improved = false;
+        if (improve) improved = cas64(tags + q1, tq, (tq & ~15ULL) | (unsigned long long)depth);
+        if (claimed) {
+            // not ready: nobody else touches the entry until the key is
+            // complete and the ready bit is published
+            unsigned long long *kw =
+                reinterpret_cast<unsigned long long *>(P.dd_key + (bucket * kDedupBucket + victim) * 16);
+            for (int w = 0; w < nw2; ++w) st64(kw + w, key_pair(L, lane, w, nw));
+        }
+        complete_before();
+        if (improve) {
+            if (improved) res = kDedupImproved;
+            else retry = true;
+        }
+        if (claimed) {
+            st64(tags + victim, ident | kTagReady | (unsigned long long)depth);
+            res = kDedupInserted;
+            count_active(&L.traffic[4], __popcll(__builtin_amdgcn_read_exec()));
+            count_active(&L.traffic[5], __popcll(__ballot(live)));
+        } else if (claim) {
+            retry = true;
+        }
+
+        // a lane with another key of its identity in the bucket: compare
+        // every candidate, then insert
+        if (seq) {
 #pragma unroll 1
             while (cand && res == kDedupPending && !retry) {
                 const int q = __ffs(cand) - 1;
                 cand &= cand - 1;
-                const unsigned long long tq = pick_tag(t, q);
-                // the key as 64-bit words, all loads in flight at once (an odd
-                // word count is padded with a zero word on both sides)
+                const unsigned long long tqs = pick_tag(t, q);
                 const unsigned long long *kw = reinterpret_cast<const unsigned long long *>(keys + q * 16);
-                const int nw2 = (nw + 1) >> 1;
-                unsigned long long kv[8];
 #pragma unroll
                 for (int w = 0; w < 8; ++w) kv[w] = w < nw2 ? ld_relaxed(kw + w) : 0ULL;
-                bool eq = true;
+                bool e = true;
 #pragma unroll
                 for (int w = 0; w < 8; ++w)
-                    if (w < nw2) eq &= kv[w] == key_pair(L, lane, w, nw);
+                    if (w < nw2) e &= kv[w] == key_pair(L, lane, w, nw);
                 count_active(&L.traffic[3], __popcll(__builtin_amdgcn_read_exec()));
-                if (!eq) continue;
-                // The snapshot tag decides only if it still holds after the
-                // key was read: an entry evicted and rewritten meanwhile (by
-                // a key of the same 27-bit identity) changes the tag word, and
-                // the lookup starts over instead of judging this key by the
-                // depth of the one the snapshot described.  What remains is
-                // an ABA of the whole tag word (same identity, same depth,
-                // ready again) inside the window, in which case the entry
-                // holds this key at that depth or a key written after it.
-                if (ld_relaxed(tags + q) != tq) {
+                if (!e) continue;
+                if (ld_relaxed(tags + q) != tqs) {
                     retry = true;
                     continue;
                 }
-                if ((int)(tq & 15ULL) <= depth) {
+                if ((int)(tqs & 15ULL) <= depth) {
                     res = kDedupDominated;
-                } else if (cas64(tags + q, tq, (tq & ~15ULL) | (unsigned long long)depth)) {
+                } else if (cas64(tags + q, tqs, (tqs & ~15ULL) | (unsigned long long)depth)) {
                     res = kDedupImproved;
                 } else {
                     retry = true;
                 }
             }
-            // 2. not stored: insert into a free entry or evict
             if (res == kDedupPending && !retry) {
                 if (busy) {
-                    retry = true;  // the same identity is being written
+                    retry = true;
                 } else {
-                    int victim = -1;
+                    int v = free_victim(t);
+                    bool lv = false;
+                    if (v < 0) {
 #pragma unroll
-                    for (int q = kDedupBucket - 1; q >= 0; --q)
-                        if (q < width && (kServe ? t[q] == 0ULL : (t[q] >> 48) != epoch)) victim = q;
-                    bool live = false;
-                    if (victim < 0) {
-                        // an entry of a configuration already found
-                        uint32_t found_mask = 0;
-#pragma unroll
-                        for (int q = 0; q < kDedupBucket; ++q) {
-                            if (q < width && (t[q] & kTagReady) && !((same_ident >> q) & 1u)) {
-                                const unsigned long long c = (t[q] >> 32) & 0xFFFFu;
-                                uint32_t f;
-                                if (kServe) {
-                                    const unsigned long long st = ld_relaxed(&P.slot_state[c]);
-                                    f = ((st >> 2) != (t[q] >> 48) || (st & kSlotDone)) ? 1u : 0u;
-                                } else {
-                                    f = (uint32_t)__hip_atomic_load(&P.results[c].found, __ATOMIC_RELAXED,
-                                                                    __HIP_MEMORY_SCOPE_AGENT);
-                                }
-                                found_mask |= f ? (1u << q) : 0u;
-                            }
-                        }
-                        if (found_mask) {
-                            victim = __ffs(found_mask) - 1;
-                        } else {
-                            // the deepest entry not shallower than the child
-                            int vd = depth - 1;
-#pragma unroll
-                            for (int q = 0; q < kDedupBucket; ++q) {
-                                const int dq = (int)(t[q] & 15ULL);
-                                if (q < width && (t[q] & kTagReady) && !((same_ident >> q) & 1u) && dq > vd) {
-                                    vd = dq;
-                                    victim = q;
-                                }
-                            }
-                            live = true;
-                        }
+                        for (int q = 0; q < kDedupBucket; ++q)
+                            kv[q] = (q < width && (t[q] & kTagReady)) ? entry_done_word<kServe>(P, t[q]) : 0ULL;
+                        v = full_victim(t, kv, same_ident, lv);
                     }
-                    if (victim < 0) {
+                    if (v < 0) {
                         res = kDedupUninserted;
-                    } else if (cas64(tags + victim, pick_tag(t, victim), ident | (unsigned long long)depth)) {
-                        // not ready: nobody else touches the entry until the
-                        // key is complete and the ready bit is published
+                    } else if (cas64(tags + v, pick_tag(t, v), ident | (unsigned long long)depth)) {
                         unsigned long long *kw =
-                            reinterpret_cast<unsigned long long *>(P.dd_key + (bucket * kDedupBucket + victim) * 16);
-                        for (int w = 0; w < ((nw + 1) >> 1); ++w) st64(kw + w, key_pair(L, lane, w, nw));
+                            reinterpret_cast<unsigned long long *>(P.dd_key + (bucket * kDedupBucket + v) * 16);
+                        for (int w = 0; w < nw2; ++w) st64(kw + w, key_pair(L, lane, w, nw));
                         complete_before();
-                        st64(tags + victim, ident | kTagReady | (unsigned long long)depth);
+                        st64(tags + v, ident | kTagReady | (unsigned long long)depth);
                         res = kDedupInserted;
                         count_active(&L.traffic[4], __popcll(__builtin_amdgcn_read_exec()));
-                        count_active(&L.traffic[5], __popcll(__ballot(live)));
+                        count_active(&L.traffic[5], __popcll(__ballot(lv)));
                     } else {
                         retry = true;
                     }
                 }
             }
-            if (retry && ++rounds >= kMaxDedupRounds) res = kDedupUninserted;
         }
+        if (retry && ++rounds >= kMaxDedupRounds) res = kDedupUninserted;
     }
     return res;
 }
