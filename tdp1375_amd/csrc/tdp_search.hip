@@ -93,6 +93,13 @@ __device__ __forceinline__ void wave_sync() {
 }
 
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+// Lane 0's value as a wave-uniform (scalar) value: the broadcast of what lane
+// 0 alone computed, which keeps values derived from it (stack bounds, loop
+// counts, masks) in scalar registers.
+__device__ __forceinline__ int lane0(int v) { return __builtin_amdgcn_readlane(v, 0); }
+__device__ __forceinline__ unsigned long long lane0_64(unsigned long long v) {
+    return ((unsigned long long)(uint32_t)lane0((int)(v >> 32)) << 32) | (uint32_t)lane0((int)(uint32_t)v);
+}
 // Fields of the wave's LDS descriptor copy as wave-uniform (scalar) values.
 __device__ __forceinline__ int cfg_int(const uint32_t &f) { return uniform((int)f); }
 __device__ __forceinline__ uint64_t cfg_bits(const uint64_t &f) {
@@ -178,7 +185,7 @@ __device__ __forceinline__ unsigned long long ctl_load(const SearchParams &P, in
 __device__ __forceinline__ unsigned long long ctl_load_uniform(const SearchParams &P, int idx) {
     unsigned long long v = 0;
     if ((threadIdx.x & 63) == 0) v = ctl_load(P, idx);
-    return ((unsigned long long)(uint32_t)__shfl((int)(v >> 32), 0) << 32) | (uint32_t)__shfl((int)v, 0);
+    return lane0_64(v);
 }
 
 __device__ __forceinline__ uint64_t fmix64(uint64_t h) {
@@ -477,7 +484,7 @@ __device__ __forceinline__ int claim_idle_wave(const SearchParams &P, const Wave
         }
         if (r >= 0) fetch_add(&P.ctrl[kCtlIdle].v, ~0ULL);
     }
-    return uniform(__shfl(r, 0));
+    return lane0(r);
 }
 
 // Hands up to `most` of the oldest nodes of this wave's stack to an idle wave.
@@ -506,7 +513,7 @@ __device__ __forceinline__ uint32_t share_nodes(const SearchParams &P, WaveState
 __device__ __forceinline__ bool receive_nodes(const SearchParams &P, WaveState &W) {
     unsigned long long cnt = 0;
     if (W.lane == 0) cnt = ld_relaxed(inbox_line(P, W.wave));
-    const int k = uniform(__shfl((int)cnt, 0));
+    const int k = lane0((int)cnt);
     if (k == 0) return false;
     complete_after();
     const int nw = (int)P.node_words;
@@ -564,10 +571,9 @@ __device__ __forceinline__ uint32_t donate_nodes(const SearchParams &P, WaveStat
             __builtin_amdgcn_s_sleep(1);
         }
     }
-    got = (uint32_t)__shfl((int)got, 0);
+    got = (uint32_t)lane0((int)got);
     if (got == 0) return 0;
-    ticket = ((unsigned long long)(unsigned)__shfl((int)(ticket >> 32), 0) << 32) |
-             (unsigned)__shfl((int)(ticket & 0xFFFFFFFFu), 0);
+    ticket = lane0_64(ticket);
     for (uint32_t g = 0; g < got; ++g) {
         const size_t slot = (size_t)((ticket + g) & (P.pool_cap - 1));
         int late = 0;
@@ -577,7 +583,7 @@ __device__ __forceinline__ uint32_t donate_nodes(const SearchParams &P, WaveStat
                 __builtin_amdgcn_s_sleep(2);
             }
         }
-        if (__shfl(late, 0)) { set_watchdog(P, W); return 0; }
+        if (lane0(late)) { set_watchdog(P, W); return 0; }
         complete_after();
         const uint32_t *src = stack_slot(P, W, W.lo + g);
         uint32_t *dst = P.pool_nodes + slot * P.node_words;
@@ -605,10 +611,9 @@ __device__ __forceinline__ bool take_one(const SearchParams &P, WaveState &W, Wa
             if (cas64(&P.ctrl[kCtlPoolHead].v, h, h + 1)) { ticket = h; ok = 1; break; }
         }
     }
-    ok = __shfl(ok, 0);
+    ok = lane0(ok);
     if (!ok) return false;
-    ticket = ((unsigned long long)(unsigned)__shfl((int)(ticket >> 32), 0) << 32) |
-             (unsigned)__shfl((int)(ticket & 0xFFFFFFFFu), 0);
+    ticket = lane0_64(ticket);
     const size_t slot = (size_t)(ticket & (P.pool_cap - 1));
     int late = 0;
     if (W.lane == 0) {
@@ -617,7 +622,7 @@ __device__ __forceinline__ bool take_one(const SearchParams &P, WaveState &W, Wa
             __builtin_amdgcn_s_sleep(2);
         }
     }
-    if (__shfl(late, 0)) { set_watchdog(P, W); return false; }
+    if (lane0(late)) { set_watchdog(P, W); return false; }
     complete_after();
     const uint32_t *src = P.pool_nodes + slot * P.node_words;
     if (W.lane < (int)P.node_words) L.node[0][W.lane] = ld32(src + W.lane);
@@ -1227,23 +1232,23 @@ __device__ __forceinline__ void flush_stage(const SearchParams &P, WaveState &W,
             PHASE_ADD(L, kPhasePush);
         } else {
             PHASE_T0();
-            int bb = wave_max(admit ? (int)(st >> 24) : -1);
-            // the next child to push, in push order: children of lower nodes
-            // of the batch first (highest node index), so that the top node's
-            // children end on top
+            // The push order is stage order: the batch's nodes are expanded
+            // bottom node first (highest node index), each staging its
+            // children in descending triple order, so the lowest admitted
+            // lane is the next child to push and the top node's children end
+            // on top.  The mask stays wave-uniform (scalar registers).
             auto pick = [&]() -> int {
-                unsigned long long grp = pm & __ballot((int)(st >> 24) == bb);
-                while (grp == 0) {
-                    --bb;
-                    grp = pm & __ballot((int)(st >> 24) == bb);
-                }
-                const int src = __ffsll((long long)grp) - 1;
-                pm &= ~(1ULL << src);
+                const int src = uniform(__ffsll((long long)pm) - 1);
+                pm &= pm - 1;
                 return src;
             };
             // two children per store when a node fits in half a wave: lanes
             // 0..31 write the first, lanes 32..63 the second
             const bool pairs = (int)P.node_words <= kWave / 2;
+            // the stack bounds are wave-uniform; said so here, the push loop's
+            // control stays in scalar registers
+            W.hi = (uint32_t)uniform((int)W.hi);
+            W.lo = (uint32_t)uniform((int)W.lo);
             while (pm) {
                 const int src0 = pick();
                 const bool two = pairs && pm != 0 && W.hi - W.lo + 2 <= P.stack_cap;
@@ -1296,7 +1301,7 @@ __device__ __forceinline__ void report_found(const SearchParams &P, WaveState &W
             fetch_add(&P.ctrl[kCtlResolved].v, 1ULL);
         }
     }
-    won = __shfl(won, 0);
+    won = lane0(won);
     if (won) {
         const int pd = (int)(nd[1] & 0xFF);
         uint32_t mv = 0;
@@ -1365,7 +1370,7 @@ __device__ __forceinline__ void serve_flush(const SearchParams &P, WaveState &W,
         }
     }
     wave_sync();
-    if (__shfl(drained, 0)) serve_drained(P, W, W.cur_cfg);
+    if (lane0(drained)) serve_drained(P, W, W.cur_cfg);
 }
 
 // Children of an expansion that needs a gain of +2.  Such a child either takes
@@ -1660,7 +1665,7 @@ __device__ __forceinline__ bool should_stop(const SearchParams &P) {
         // a serving session has no fixed set of configurations to resolve
         if (!kServe) s |= ctl_load(P, kCtlResolved) >= (unsigned long long)P.num_configs;
     }
-    return __shfl(s, 0) != 0;
+    return lane0(s) != 0;
 }
 
 // Serving: the slot's descriptor into the wave's LDS copy, read with
@@ -1682,7 +1687,7 @@ __device__ __forceinline__ bool serve_switch(const SearchParams &P, WaveState &W
                                              int cfg) {
     unsigned long long st = 0;
     if (W.lane == 0) st = ld_relaxed(&P.slot_state[cfg]);
-    const uint32_t lo = (uint32_t)__shfl((int)(uint32_t)st, 0);
+    const uint32_t lo = (uint32_t)lane0((int)(uint32_t)st);
     const uint32_t gen = lo >> 2;
     if (cfg != W.cur_cfg || gen != W.cur_gen) {
         flush_counters<true>(P, W, L);
@@ -1706,7 +1711,7 @@ __device__ __forceinline__ int serve_claim(const SearchParams &P, const WaveStat
             slot = (int)ld32(&P.ring_slot[next % P.ring]);
         }
     }
-    return uniform(__shfl(slot, 0));
+    return lane0(slot);
 }
 
 // Serving: the feeder wave.  Copies each request the host publishes in its
@@ -1743,16 +1748,16 @@ __device__ __forceinline__ void serve_feeder(const SearchParams &P, WaveState &W
                 }
             }
         }
-        if (__shfl(host_stop, 0)) {
+        if (lane0(host_stop)) {
             if (W.lane == 0) __hip_atomic_store(&P.ctrl[kCtlStop].v, 3ULL, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             break;
         }
-        if (__shfl(dev_stop, 0)) break;
+        if (lane0(dev_stop)) break;
         if (past_deadline(W)) {
             set_watchdog(P, W);
             break;
         }
-        seq = ((unsigned long long)(uint32_t)__shfl((int)(seq >> 32), 0) << 32) | (uint32_t)__shfl((int)seq, 0);
+        seq = lane0_64(seq);
         // at most one ring of published requests not yet claimed (the
         // claimers read their slot from ring_slot)
         const bool ahead = next - ctl_load_uniform(P, kCtlNextConfig) >= (unsigned long long)P.ring;
@@ -1841,6 +1846,8 @@ __device__ __forceinline__ void search_body(const SearchParams &P) {
     int backoff = 1;
 
     while (true) {
+        W.hi = (uint32_t)uniform((int)W.hi);
+        W.lo = (uint32_t)uniform((int)W.lo);
         if (W.stop || ((++polls & 7) == 0 && should_stop<kServe>(P))) break;
         if ((polls & 63) == 0 && past_deadline(W)) {
             set_watchdog(P, W);
@@ -1887,7 +1894,7 @@ __device__ __forceinline__ void search_body(const SearchParams &P) {
                                                        __HIP_MEMORY_SCOPE_AGENT) & bit) ? 1 : 0;
                         if (mine) fetch_add(&P.ctrl[kCtlIdle].v, ~0ULL);
                     }
-                    if (!__shfl(mine, 0)) {
+                    if (!lane0(mine)) {
                         // a donor has claimed this wave: its nodes are on the
                         // way to the inbox (the donor took the wave out of the
                         // idle count)
@@ -1912,7 +1919,7 @@ __device__ __forceinline__ void search_body(const SearchParams &P) {
                 if (W.lane == 0) L.takes = 1;
             } else {
                 if (W.lane == 0) c = (int)fetch_add(&P.ctrl[kCtlNextConfig].v, 1ULL);
-                c = uniform(__shfl(c, 0));
+                c = lane0(c);
                 if (c >= P.num_configs) {
                     W.configs_left = false;
                     continue;
@@ -2005,8 +2012,8 @@ __device__ __forceinline__ void search_body(const SearchParams &P) {
                         if (work) fetch_add(&P.ctrl[kCtlIdle].v, ~0ULL);
                     }
                 }
-                if (__shfl(done, 0)) break;
-                if (__shfl(work, 0)) {
+                if (lane0(done)) break;
+                if (lane0(work)) {
                     idle = false;   // a failed take re-enters the idle set
                     have = take_one(P, W, L);
                     PHASE_ADD(L, kPhaseShare);
