@@ -215,6 +215,18 @@ TDP_API double tdp_filter_last_kernel_ms(void);
  * cycle analysis and O(1) parity delta. out: C(n,3) ints. */
 TDP_API int tdp_eval_children(const uint8_t *pi, const uint8_t *spi, int n, int32_t *out);
 
+/* Validation entry (extension, no reference counterpart): the cycle-local
+ * enumeration of two nodes a and b (same n, 3..32) that the search kernel
+ * runs in one pass when both admit only +2 children, and the same two nodes
+ * enumerated one after the other.  goal_x / admit_x: bit 2 set when a +2
+ * child of node x is a goal / admissible.  out: 2 records of 4 + 512 ints
+ * (status 1 = done, goal node (1 = a, 0 = b, -1 none), goal triple index,
+ * staged count, stage entries i | j << 6 | k << 12 | odd cycles << 18 |
+ * node << 24); record 0 the paired pass, record 1 the sequential passes. */
+TDP_API int tdp_eval_pair(const uint8_t *pi_a, const uint8_t *spi_a, const uint8_t *pi_b, const uint8_t *spi_b,
+                          int n, int e_a, int e_b, uint32_t goal_a, uint32_t admit_a, uint32_t goal_b,
+                          uint32_t admit_b, int32_t *out);
+
 /* Validation entry (extension): the learned ordering's 128 feature bits of
  * each of the C(n,3) moves of the node (pi, spi) at depth `depth` of a search
  * to max_depth with start odd-cycle count e0, as the search kernel computes

@@ -34,6 +34,7 @@ EXPORTED_SYMBOLS = (
     "tdp_last_error", "tdp_init_slots", "tdp_destroy_slots", "tdp_search", "tdp_search_ex",
     "tdp_search_batch", "tdp_last_launch_stats", "tdp_cancel", "tdp_filter", "tdp_eval_features",
     "tdp_filter_last_kernel_ms", "tdp_filter_noalloc", "tdp_eval_children", "tdp_host_eval_children",
+    "tdp_eval_pair",
     "tdp_host_predicate_tables", "tdp_set_tuning", "tdp_set_launch_deadline", "tdp_set_expansion_batch",
     "tdp_canonical_batch",
     "tdp_set_alloc_timeout", "tdp_alloc_retries",
@@ -140,6 +141,7 @@ def load() -> ctypes.CDLL:
         L.tdp_alloc_retries.argtypes = []
         L.tdp_alloc_retries.restype = ctypes.c_uint64
         L.tdp_eval_children.argtypes = [u8p, u8p, ctypes.c_int, i32p]
+        L.tdp_eval_pair.argtypes = [u8p, u8p, u8p, u8p] + [ctypes.c_int] * 3 + [ctypes.c_uint32] * 4 + [i32p]
         L.tdp_host_eval_children.argtypes = [u8p, u8p, ctypes.c_int, i32p]
         L.tdp_eval_features.argtypes = [u8p, u8p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                         ctypes.POINTER(ctypes.c_uint32)]

@@ -1939,6 +1939,32 @@ int tdp_eval_children(const uint8_t *pi, const uint8_t *spi, int n, int32_t *out
     });
 }
 
+int tdp_eval_pair(const uint8_t *pi_a, const uint8_t *spi_a, const uint8_t *pi_b, const uint8_t *spi_b, int n,
+                  int e_a, int e_b, uint32_t goal_a, uint32_t admit_a, uint32_t goal_b, uint32_t admit_b,
+                  int32_t *out) {
+    return guarded([&] {
+        if (n < 3 || n > 32) throw std::invalid_argument("tdp_eval_pair: n must be in 3..32");
+        if (!out) throw std::invalid_argument("null argument");
+        CaseInput ca{pi_a, spi_a, n, 0, 1, 0, 1.0f}, cb{pi_b, spi_b, n, 0, 1, 0, 1.0f};
+        validate_case(ca);
+        validate_case(cb);
+        const size_t rec = 4 + (size_t)kStageEntries;
+        DevPtr d_in = make_dev(256, "eval.pair.in"), d_out = make_dev(2 * rec * 4, "eval.pair.out");
+        uint8_t host[256] = {0};
+        std::memcpy(host, pi_a, (size_t)n);
+        std::memcpy(host + 64, spi_a, (size_t)n);
+        std::memcpy(host + 128, pi_b, (size_t)n);
+        std::memcpy(host + 192, spi_b, (size_t)n);
+        HIP_CHECK(hipMemcpy(d_in.get(), host, 256, hipMemcpyHostToDevice));
+        const uint8_t *b = as<uint8_t>(d_in);
+        hipLaunchKernelGGL(eval_pair_kernel, dim3(1), dim3(64), 0, nullptr, b, b + 64, b + 128, b + 192, n, e_a, e_b,
+                           goal_a, admit_a, goal_b, admit_b, as<int32_t>(d_out));
+        HIP_CHECK(hipGetLastError());
+        HIP_CHECK(hipDeviceSynchronize());
+        HIP_CHECK(hipMemcpy(out, d_out.get(), 2 * rec * 4, hipMemcpyDeviceToHost));
+    });
+}
+
 int tdp_eval_features(const uint8_t *pi, const uint8_t *spi, int n, int e0, int depth, int max_depth,
                       uint32_t *out) {
     return guarded([&] {

@@ -11,6 +11,8 @@ namespace tdp {
 
 // Most nodes a wave expands before one dedup pass over all their children.
 constexpr int kExpansionBatchMax = 8;
+// Children staged per wave before a dedup pass.
+constexpr int kStageEntries = 512;
 
 struct SearchParams {
     const ConfigDesc *cfgs;
@@ -77,6 +79,9 @@ __global__ void search_learned_kernel(SearchParams P);
 __global__ void serve_kernel(SearchParams P);
 __global__ void serve_learned_kernel(SearchParams P);
 __global__ void reset_kernel(SearchParams P);
+__global__ void eval_pair_kernel(const uint8_t *pia, const uint8_t *spia, const uint8_t *pib, const uint8_t *spib,
+                                 int n, int ea, int eb, uint32_t ga, uint32_t aa, uint32_t gb, uint32_t ab,
+                                 int32_t *out);
 __global__ void eval_children_kernel(const uint8_t *pi, const uint8_t *spi, int n,
                                      int T, int e, int32_t *out, int e0, int d, int D, uint32_t *features);
 

@@ -37,7 +37,7 @@ namespace tdp {
 namespace {
 
 constexpr int kWavesPerBlock = 4;
-constexpr int kStage = 512;
+constexpr int kStage = kStageEntries;
 constexpr int kBatchMax = kExpansionBatchMax;
 // Rounds of one dedup lookup before the child is admitted without an entry
 // (a round is lost only to a concurrent update of the same bucket).
@@ -1520,6 +1520,157 @@ __device__ __forceinline__ int cycle_local_children(WaveState &W, WaveLds &L, co
     return kLocalDone;
 }
 
+// The deltas that make a goal child (gmask) and an admissible one (amask) of
+// a node at depth d with e odd cycles: bits over delta in {-2, 0, +2} (both
+// predicates are monotone in the odd-cycle count and a move changes it by -2,
+// 0 or +2).  Root mode admits goals and live roots alike.
+__device__ __forceinline__ void child_masks(const ConfigDesc &C, int d, int e, bool root_mode, uint32_t &gmask,
+                                            uint32_t &amask) {
+    const int D = cfg_int(C.max_depth);
+    const uint64_t acc = cfg_bits(C.accept[d + 1]);
+    const uint64_t rch = (d + 1 < D) ? cfg_bits(C.reach[d + 1]) : 0ULL;
+    gmask = 0;
+    amask = 0;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        const int ev = e + 2 * q - 2;
+        if (ev >= 0 && ev < 64) {
+            gmask |= (uint32_t)((acc >> ev) & 1ULL) << q;
+            amask |= (uint32_t)((rch >> ev) & 1ULL) << q;
+        }
+    }
+    if (root_mode) {
+        amask |= gmask;
+        gmask = 0;
+    }
+}
+
+// Two nodes of a batch (L.node[ba], L.node[bb], n <= 32) expanded together
+// when only delta = +2 qualifies for both: lanes 0..31 hold the symbols of
+// node ba, lanes 32..63 those of node bb, so one pass of pointer jumping
+// analyses both and one chunk loop enumerates the cycle-local children of
+// both (node ba's candidates first).  The result is that of
+// cycle_local_children on ba and then bb: ba's admitted children staged
+// before bb's, each node's in descending triple order; a goal of ba ends the
+// enumeration before bb's children count, a goal of bb drops ba's staged
+// children.  Returns kLocalNo (nothing done: an even cycle, too many
+// candidates for the pair or for either node) or kLocalFull (nothing done:
+// no room behind scount), else kLocalDone with goal_b / goal_t set on a goal.
+__device__ __forceinline__ int pair_local_children(WaveState &W, WaveLds &L, int ba, int bb, int n, int sw, int T,
+                                                   int ea, int eb, uint32_t ga, uint32_t aa, uint32_t gb,
+                                                   uint32_t ab, int &scount, int &goal_b, int &goal_t) {
+    const int l = W.lane, half = l >> 5, s = l & 31, base = l & 32;
+    // cycle analysis of both nodes (build_posinfo's pointer jumping, lane
+    // indices as symbols: a cycle of node bb lies in lanes 32..63)
+    const uint8_t *spi = reinterpret_cast<const uint8_t *>(L.node[half ? bb : ba] + kNodeHeaderWords);
+    const uint8_t *pi = spi + 4 * sw;
+    const int nxt = (s < n) ? base + (int)spi[s] : l;
+    int w = l, p = nxt;
+    for (int r = 0; r < 5; ++r) {
+        const int wo = __shfl(w, p);
+        if (__ballot((wo & 0xFF) != (w & 0xFF)) == 0) break;
+        const int pp = __shfl(p, p);
+        if ((wo & 0xFF) < (w & 0xFF)) w = wo + (1 << (8 + r));
+        p = pp;
+    }
+    const int root = w & 0xFF;
+    const int f = w >> 8;
+    const int f_next = __shfl(f, nxt);
+    const int len = __shfl(f_next + 1, root);
+    const int pos = (f == 0) ? 0 : len - f;
+    const bool is_root = s < n && root == l;
+    if (__ballot(is_root && (len & 1) == 0) != 0) return kLocalNo;   // even cycles: one node at a time
+    const bool big = is_root && len >= 3;
+    const bool listed = is_root && len >= 2;
+    // candidates and members over both nodes (node ba's first)
+    const int ka = wave_scan_add(big ? choose3(len) : 0);
+    const int mo = wave_scan_add(listed ? len : 0);
+    const int KA = __builtin_amdgcn_readlane(ka, 31);
+    const int K = __builtin_amdgcn_readlane(ka, 63);
+    const int KB = K - KA;
+    if (K > kCycleLocalMax || 4 * KA > T || 4 * KB > T) return kLocalNo;
+    if (scount + K > kStage) return kLocalFull;
+    const unsigned long long roots_a = __ballot(big);
+    const int moff = mo - (listed ? len : 0);
+    const uint32_t acw = (uint32_t)(ka - (big ? choose3(len) : 0)) | ((uint32_t)len << 16) | ((uint32_t)moff << 24);
+    // member entries: cycle info (root lane, position, length) with the
+    // symbol's position in its node's pi in bits 24..29
+    const int pos_in_pi = __builtin_amdgcn_ds_permute((s < n ? base + (int)pi[s] : l) << 2, s);
+    const int my_moff = __shfl(moff, root);
+    if (s < n && len >= 2) L.memb[my_moff + pos] = cycle_info(root, pos, len) | ((uint32_t)pos_in_pi << 24);
+    wave_sync();
+    uint32_t *tmp_k = &L.ckey[0][0];
+    uint32_t *tmp_e = &L.ckey[8][0];
+    int best = -1, cnt = 0;
+    for (int g0 = 0; g0 < K; g0 += kWave) {
+        const int g = g0 + l;
+        const bool valid = g < K;
+        const bool inb = g >= KA;
+        uint32_t ma, mb, mc;
+        if (!valid) {
+            // three fixed points at distinct positions, result masked below
+            ma = cycle_info(0, 0, 1);
+            mb = cycle_info(1, 0, 1) | (1u << 24);
+            mc = cycle_info(2, 0, 1) | (2u << 24);
+        } else {
+            // the cycle holding candidate g: the last one starting at or
+            // before it (node bb's cycles start at KA or later)
+            uint32_t cw = 0;
+            for (unsigned long long rm = roots_a; rm; rm &= rm - 1) {
+                const uint32_t cv = (uint32_t)__builtin_amdgcn_readlane((int)acw, __ffsll((long long)rm) - 1);
+                cw = ((int)(cv & 0xFFFFu) <= g) ? cv : cw;
+            }
+            const uint32_t tri = W.colex3[g - (int)(cw & 0xFFFFu)];
+            const int co = (int)(cw >> 24);
+            ma = L.memb[co + (int)(tri & 0xFFu)];
+            mb = L.memb[co + (int)((tri >> 8) & 0xFFu)];
+            mc = L.memb[co + (int)(tri >> 16)];
+        }
+        const uint32_t lo = min(ma, min(mb, mc)), hi = max(ma, max(mb, mc));
+        const uint32_t mid = ma ^ mb ^ mc ^ lo ^ hi;
+        const int i = (int)(lo >> 24), j = (int)(mid >> 24), k = (int)(hi >> 24);
+        const int delta = parity_delta(lo, mid, hi);
+        const uint32_t di = (uint32_t)(delta + 2) >> 1;
+        const bool goal = valid && (((inb ? gb : ga) >> di) & 1u);
+        const bool adm = valid && (((inb ? ab : aa) >> di) & 1u);
+        // descending order puts node ba's entries first
+        const int key = (inb ? 0 : (1 << 18)) | (i << 12) | (j << 6) | k;
+        if (goal) best = max(best, key);
+        const unsigned long long am = __ballot(adm);
+        const unsigned long long below = (l == 0) ? 0ULL : (am & (~0ULL >> (64 - l)));
+        if (adm) {
+            tmp_k[cnt + __popcll(below)] = (uint32_t)key;
+            tmp_e[cnt + __popcll(below)] = stage_entry(i, j, k, (inb ? eb : ea) + delta, inb ? bb : ba);
+        }
+        cnt += __popcll(am);
+    }
+    best = wave_max(best);
+    if (best >= 0) {
+        goal_b = (best >> 18) ? ba : bb;
+        goal_t = lex_index(n, T, (best >> 12) & 63, (best >> 6) & 63, best & 63);
+        return kLocalDone;
+    }
+    wave_sync();
+    if (cnt <= kWave) {
+        const uint32_t mk = l < cnt ? tmp_k[l] : 0u;
+        const uint32_t me = l < cnt ? tmp_e[l] : 0u;
+        int rank = 0;
+        for (int q = 0; q < cnt; ++q) rank += (uint32_t)__builtin_amdgcn_readlane((int)mk, q) > mk ? 1 : 0;
+        wave_sync();
+        if (l < cnt) L.stage[scount + rank] = me;
+    } else {
+        for (int idx = l; idx < cnt; idx += kWave) {
+            const uint32_t kk = tmp_k[idx];
+            int rank = 0;
+            for (int q = 0; q < cnt; ++q) rank += tmp_k[q] > kk ? 1 : 0;
+            L.stage[scount + rank] = tmp_e[idx];
+        }
+    }
+    wave_sync();
+    scount += cnt;
+    return kLocalDone;
+}
+
 // Evaluates every child of node nd (index b of the batch in L.node) and
 // stages the admissible ones behind the scount entries already staged,
 // flushing the stage only when it would overflow.  Normal mode: returns the
@@ -2080,6 +2231,39 @@ __device__ __forceinline__ void search_body(const SearchParams &P) {
             lds_add(&L.cnt[kCntExpansions], 1);
             lds_add(&L.cnt[kCntGenerated], (unsigned long long)cfg_int(C.triple_count));
             ++expanded;
+            if (!kLearned && b >= 1 && cfg_int(C.n) <= kWave / 2) {
+                // the next node too when it is expandable and both take the
+                // +2-only enumeration: one pass for the pair
+                const uint32_t *nd2 = L.node[b - 1];
+                const int d2 = uniform((int)(nd2[1] & 0xFF));
+                const int e2 = uniform((int)((nd2[1] >> 8) & 0xFF));
+                uint32_t ga, aa, gb, ab;
+                child_masks(C, d, e, false, ga, aa);
+                child_masks(C, d2, e2, false, gb, ab);
+                if ((ga | aa) == 4u && (gb | ab) == 4u && !((cfg_bits(C.accept[d2]) >> e2) & 1ULL) &&
+                    d2 < cfg_int(C.max_depth) && ((cfg_bits(C.reach[d2]) >> e2) & 1ULL)) {
+                    PHASE_T0();
+                    int goal_b = -1, goal_t = -1;
+                    const int r = pair_local_children(W, L, b, b - 1, cfg_int(C.n), (int)P.spi_words,
+                                                      cfg_int(C.triple_count), e, e2, ga, aa, gb, ab, scount,
+                                                      goal_b, goal_t);
+                    PHASE_ADD(L, kPhaseScan);
+                    if (r == kLocalDone) {
+                        lds_add(&L.cnt[kCntExpansions], 1);
+                        lds_add(&L.cnt[kCntGenerated], (unsigned long long)cfg_int(C.triple_count));
+                        ++expanded;
+                        --b;
+                        rest = b;
+                        if (goal_t >= 0) {
+                            report_found<kServe>(P, W, L.node[goal_b], cfg, (goal_b == b ? d2 : d) + 1, goal_t);
+                            terminal = true;
+                            break;
+                        }
+                        if (scount + (int)(W.hi - hi0) >= kStage / 2) break;
+                        continue;
+                    }
+                }
+            }
             const int goal = expand_stage<kLearned, kServe>(P, W, L, C, nd, b, false, scount);
             if (goal >= 0) {
                 report_found<kServe>(P, W, nd, cfg, d + 1, goal);
@@ -2214,6 +2398,67 @@ __global__ __launch_bounds__(64) void eval_children_kernel(const uint8_t *pi, co
         }
         return kScanGo;
     });
+}
+
+// Validation: the cycle-local enumeration of two nodes (index 1 = a, 0 = b
+// of a batch, same n <= 32, +2-only masks) by the paired pass and by one
+// pass per node, as the batch loop would run them.  out[0..4) and
+// out[4..4 + kStage) of each record: status, goal node, goal triple, staged
+// count, then the stage entries; record 0 is the pair, record 1 the
+// sequential passes.
+__global__ __launch_bounds__(64) void eval_pair_kernel(const uint8_t *pia, const uint8_t *spia, const uint8_t *pib,
+                                                      const uint8_t *spib, int n, int ea, int eb, uint32_t ga,
+                                                      uint32_t aa, uint32_t gb, uint32_t ab, int32_t *out) {
+    __shared__ WaveLds L;
+    __shared__ uint16_t colex[kColexPairs];
+    __shared__ uint32_t colex3[kCycleLocalMax];
+    fill_colex(colex, colex3);
+    WaveState W;
+    W.lane = threadIdx.x;
+    W.colex = colex;
+    W.colex3 = colex3;
+    const int sw = spi_words_for(kMaxN);
+    const int T = n * (n - 1) * (n - 2) / 6;
+    for (int b = 0; b < 2; ++b) {
+        const uint8_t *pi = b ? pia : pib, *spi = b ? spia : spib;
+        if (W.lane < sw) {
+            uint32_t w = 0, v = 0;
+            for (int q = 0; q < 4; ++q) {
+                const int s = 4 * W.lane + q;
+                if (s < n) w |= (uint32_t)spi[s] << (8 * q);
+                if (s < n) v |= (uint32_t)pi[s] << (8 * q);
+            }
+            L.node[b][kNodeHeaderWords + W.lane] = w;
+            L.node[b][kNodeHeaderWords + sw + W.lane] = v;
+        }
+    }
+    wave_sync();
+    for (int rec = 0; rec < 2; ++rec) {
+        int32_t *o = out + rec * (4 + kStage);
+        int scount = 0, goal_b = -1, goal_t = -1, st;
+        if (rec == 0) {
+            st = pair_local_children(W, L, 1, 0, n, sw, T, ea, eb, ga, aa, gb, ab, scount, goal_b, goal_t);
+        } else {
+            uint32_t si = build_posinfo(L, L.node[1], 0, n, sw, W.lane);
+            st = cycle_local_children(W, L, L.node[1], 1, 0, si, n, sw, T, ea, ga, aa, false, scount, goal_t);
+            if (st == kLocalDone && goal_t >= 0) {
+                goal_b = 1;
+            } else if (st == kLocalDone) {
+                si = build_posinfo(L, L.node[0], 0, n, sw, W.lane);
+                st = cycle_local_children(W, L, L.node[0], 0, 0, si, n, sw, T, eb, gb, ab, false, scount, goal_t);
+                if (goal_t >= 0) goal_b = 0;
+            }
+        }
+        wave_sync();
+        if (W.lane == 0) {
+            o[0] = st;
+            o[1] = goal_b;
+            o[2] = goal_t;
+            o[3] = scount;
+        }
+        for (int q = W.lane; q < scount; q += kWave) o[4 + q] = (int32_t)L.stage[q];
+        wave_sync();
+    }
 }
 
 }  // namespace tdp

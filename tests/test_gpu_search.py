@@ -70,6 +70,73 @@ def test_eval_children_matches_recount(gpu_lib):
         assert np.array_equal(got, want), f"n={n} it={it}"
 
 
+def _plus2_stage(pi, spi, e, goal, admit, node):
+    """The cycle-local enumeration's result for one node from the oracle's
+    recount: the largest +2 triple index if +2 children are goals, else the
+    admitted +2 children in descending triple order as stage entries."""
+    n = len(pi)
+    odd = oracle.children_even(pi, spi)
+    triples = [(i, j, k) for i in range(n) for j in range(i + 1, n) for k in range(j + 1, n)]
+    plus2 = [t for t in range(len(triples)) if odd[t] == e + 2]
+    if goal and plus2:
+        return max(plus2), []
+    if not admit:
+        return -1, []
+    return -1, [i | (j << 6) | (k << 12) | ((e + 2) << 18) | (node << 24)
+                for (i, j, k) in (triples[t] for t in sorted(plus2, reverse=True))]
+
+
+def test_paired_enumeration_matches_sequential_and_recount(gpu_lib):
+    """Two batch nodes enumerated in one pass (n <= 32, +2-only) give the
+    staged children and goal of one pass per node, and both equal the
+    oracle's recount."""
+    import ctypes
+    rng = np.random.default_rng(23)
+    rec = 4 + 512
+    paired = 0
+    for it in range(150):
+        n = int(rng.integers(3, 33))
+        nodes = []
+        for _ in range(2):
+            sym = [int(x) for x in rng.permutation(n)]
+            cyc, p = [], 0
+            even = it % 10 == 9
+            while p < n:
+                lengths = [x for x in ([1, 3, 5, 7] if not even else [1, 2, 3, 4, 5]) if x <= n - p]
+                L = int(rng.choice(lengths))
+                cyc.append(sym[p:p + L])
+                p += L
+            spi = W.one_line(cyc, n)
+            pi = [int(x) for x in rng.permutation(n)]
+            nodes.append((pi, spi, W.odd_cycle_count(spi)))
+        (pa, sa, ea), (pb, sb, eb) = nodes
+        ga, aa, gb, ab = [4 * int(x) for x in rng.integers(0, 2, 4)]
+        aa = 4 if ga == 0 else aa
+        ab = 4 if gb == 0 else ab
+        out = np.zeros(2 * rec, np.int32)
+        T._lib.check(gpu_lib.tdp_eval_pair(T._lib.u8(pa, 64), T._lib.u8(sa, 64), T._lib.u8(pb, 64),
+                                           T._lib.u8(sb, 64), n, ea, eb, ga, aa, gb, ab,
+                                           out.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))))
+        pr, sq = out[:rec], out[rec:]
+        if pr[0] != 1:
+            continue   # the pair declined (an even cycle, or too many candidates): one node at a time
+        paired += 1
+        assert sq[0] == 1, f"it={it}"
+        assert list(pr[1:3]) == list(sq[1:3]), f"it={it} n={n}"
+        if pr[1] < 0:   # no goal: the same children staged in the same order
+            assert pr[3] == sq[3] and list(pr[4:4 + pr[3]]) == list(sq[4:4 + sq[3]]), f"it={it} n={n}"
+        # the oracle: node a (batch index 1) first, then b (index 0)
+        ta, sta = _plus2_stage(pa, sa, ea, ga, aa, 1)
+        if ta >= 0:
+            want = (1, ta, [])
+        else:
+            tb, stb = _plus2_stage(pb, sb, eb, gb, ab, 0)
+            want = (0, tb, []) if tb >= 0 else (-1, -1, sta + stb)
+        got_stage = [] if want[0] >= 0 else [int(x) for x in pr[4:4 + pr[3]]]
+        assert (int(pr[1]), int(pr[2]), got_stage) == want, f"it={it} n={n}"
+    assert paired >= 90
+
+
 def test_single_wave_parity_random_n12(slots):
     """One wave, one configuration at a time: the reference's Q=1 order."""
     wl = W.workload_random_n12()
