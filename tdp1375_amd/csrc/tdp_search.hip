@@ -463,28 +463,35 @@ __device__ __forceinline__ unsigned long long *idle_word(const SearchParams &P, 
     return &P.idle_bits[(size_t)word * 16];
 }
 
-// Claims one idle wave; returns its index or -1 (wave-uniform).
+// Claims one idle wave; returns its index or -1 (wave-uniform).  Up to four
+// idle words are read at once (one lane each), so a busy grid costs one
+// memory round trip; the words with set bits are then tried in turn.
 __device__ __forceinline__ int claim_idle_wave(const SearchParams &P, const WaveState &W, unsigned salt) {
+    const int words = (int)P.idle_words;
+    const int tries = words < 4 ? words : 4;
+    const int w0 = (int)(((unsigned)(W.wave >> 6) + salt) % (unsigned)words);
+    int word = w0 + W.lane;
+    word = word >= words ? word - words : word;
+    unsigned long long bits = 0;
+    if (W.lane < tries) bits = ld_relaxed(idle_word(P, word));
     int r = -1;
-    if (W.lane == 0) {
-        const int words = (int)P.idle_words;
-        const int tries = words < 4 ? words : 4;
-        int word = (int)(((unsigned)(W.wave >> 6) + salt) % (unsigned)words);
-        for (int t = 0; t < tries && r < 0; ++t) {
-            unsigned long long bits = ld_relaxed(idle_word(P, word));
-            while (bits && r < 0) {
+    for (unsigned long long nz = __ballot(bits != 0); nz && r < 0; nz &= nz - 1) {
+        const int c = __ffsll((long long)nz) - 1;
+        int got = -1;
+        if (W.lane == c) {
+            while (bits && got < 0) {
                 const int b = __ffsll((long long)bits) - 1;
                 const unsigned long long bit = 1ULL << b;
                 const unsigned long long old =
                     __hip_atomic_fetch_and(idle_word(P, word), ~bit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                if (old & bit) r = word * 64 + b;
+                if (old & bit) got = word * 64 + b;
                 else bits = old & ~bit;
             }
-            word = (word + 1 == words) ? 0 : word + 1;
+            if (got >= 0) fetch_add(&P.ctrl[kCtlIdle].v, ~0ULL);
         }
-        if (r >= 0) fetch_add(&P.ctrl[kCtlIdle].v, ~0ULL);
+        r = __builtin_amdgcn_readlane(got, c);
     }
-    return lane0(r);
+    return r;
 }
 
 // Hands up to `most` of the oldest nodes of this wave's stack to an idle wave.
@@ -1999,7 +2006,7 @@ __device__ __forceinline__ void search_body(const SearchParams &P) {
     while (true) {
         W.hi = (uint32_t)uniform((int)W.hi);
         W.lo = (uint32_t)uniform((int)W.lo);
-        if (W.stop || ((++polls & 7) == 0 && should_stop<kServe>(P))) break;
+        if (W.stop || ((++polls & 31) == 0 && should_stop<kServe>(P))) break;
         if ((polls & 63) == 0 && past_deadline(W)) {
             set_watchdog(P, W);
             break;
