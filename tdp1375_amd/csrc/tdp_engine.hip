@@ -412,15 +412,24 @@ uint64_t splitmix(uint64_t &s) {
 // Launch tuning of the work-sharing mode (tdp_set_tuning); 0 = default.
 struct Tuning {
     int waves_per_cu = 0;     // default: the occupancy limit of search_kernel
-    int share_interval = 16;  // expansions between donation checks (power of two)
+    int share_interval = 8;   // expansions between donation checks (power of two)
     int share_max = 8;        // nodes per donation
     int idle_backoff = 16;    // longest idle sleep, x 127 s_sleep units
     int max_active = 1024;    // configurations per persistent launch
     int deadline_ms = 1000;   // TDP_FLAG_SOFT_DEADLINE: time budget of each launch
-    int batch = 8;            // nodes a wave expands before one dedup pass (plain ordering)
+    int batch = 12;           // nodes a wave expands before one dedup pass (plain ordering)
 };
 std::mutex g_tuning_mutex;
 Tuning g_tuning;
+
+// Nodes per expansion batch: the tuning's, at most what the kernel's LDS node
+// pool holds at this node size, and at most kLearnedBatchMax with the learned
+// ordering (which keeps each batch node's cycle table).
+int batch_nodes(int batch, int node_words, bool learned) {
+    int b = std::min(batch, kNodePoolWords / node_words);
+    if (learned) b = std::min(b, kLearnedBatchMax);
+    return std::max(b, 1);
+}
 
 Tuning current_tuning() {
     std::lock_guard<std::mutex> lk(g_tuning_mutex);
@@ -757,8 +766,9 @@ void run_launch(Slot &s, const std::vector<CaseInput> &cases, uint32_t flags, bo
     P.donate = donate ? 1 : 0;
     P.share_mask = (uint32_t)tune.share_interval - 1;
     // the fixed expansion order of the deterministic modes takes one node at
-    // a time
-    P.batch = donate ? tune.batch : 1;
+    // a time; a batch fits the kernel's LDS node pool (and, with the learned
+    // ordering, its per-node cycle tables)
+    P.batch = donate ? batch_nodes(tune.batch, nw, learned) : 1;
     P.share_max = share_max;
     P.idle_backoff = (uint32_t)tune.idle_backoff;
     P.timeout_ticks = soft ? (unsigned long long)tune.deadline_ms * 100000ULL : kernel_timeout_ticks();
@@ -1145,7 +1155,7 @@ void serve_start(int64_t queue_bytes, int64_t dedup, int slots, int max_n, uint3
     P.cancel_flag = nullptr;
     P.donate = 1;
     P.share_mask = (uint32_t)tune.share_interval - 1;
-    P.batch = tune.batch;
+    P.batch = batch_nodes(tune.batch, nw, S->learned);
     P.share_max = share_max;
     P.idle_backoff = (uint32_t)tune.idle_backoff;
     P.timeout_ticks = (unsigned long long)lifetime_s * 100000000ULL;
@@ -1684,7 +1694,7 @@ int tdp_set_launch_deadline(int milliseconds) {
 
 int tdp_set_expansion_batch(int nodes) {
     return guarded([&] {
-        if (nodes < 0 || nodes > kExpansionBatchMax) throw std::invalid_argument("batch must be in 0..8");
+        if (nodes < 0 || nodes > kExpansionBatchMax) throw std::invalid_argument("batch must be in 0..16");
         std::lock_guard<std::mutex> lk(g_tuning_mutex);
         g_tuning.batch = nodes ? nodes : Tuning().batch;
     });

@@ -9,8 +9,13 @@
 
 namespace tdp {
 
-// Most nodes a wave expands before one dedup pass over all their children.
-constexpr int kExpansionBatchMax = 8;
+// Most nodes a wave expands before one dedup pass over all their children
+// (the learned ordering, which keeps each node's cycle info, at most
+// kLearnedBatchMax); a batch also fits the LDS node pool of kNodePoolWords
+// words at the launch's node size.
+constexpr int kExpansionBatchMax = 16;
+constexpr int kLearnedBatchMax = 8;
+constexpr int kNodePoolWords = 448;
 // Children staged per wave before a dedup pass.
 constexpr int kStageEntries = 512;
 

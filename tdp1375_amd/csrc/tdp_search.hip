@@ -49,8 +49,10 @@ enum DedupOutcome { kDedupPending = -1, kDedupDominated = 0, kDedupImproved = 1,
 // Per-wave LDS slice.  The 64-bit counters live here rather than in
 // registers (lane 0 updates them with wave-uniform values).
 struct WaveLds {
-    uint32_t node[kBatchMax][48];            // the nodes being expanded (node[0] alone outside batches)
-    uint32_t posinfo[kBatchMax][64];         // cycle info per position of each node of the batch
+    uint32_t nodes[kNodePoolWords];          // the nodes being expanded, node_words apart (node 0 alone
+                                             // outside batches)
+    uint32_t posinfo[kLearnedBatchMax][64];  // cycle info per position of each node of a learned batch
+                                             // (row 0 only for the plain ordering)
     uint32_t memb[64];                       // cycle members in spi order, cycles of length >= 2:
                                              // cycle info | position in pi << 24
     uint32_t ememb[64];                      // the members of the even cycles again, contiguous
@@ -69,6 +71,15 @@ struct WaveLds {
                                              // evictions
     unsigned long long phase[kNumPhases];    // 10 ns ticks per phase (TDP_PHASE_TIMING builds)
 };
+
+// Node b of the batch in the wave's LDS pool (nodes of sw spi words are
+// node_words_for(4 sw) words apart).
+__device__ __forceinline__ uint32_t *node_at(WaveLds &L, int b, int sw) {
+    return L.nodes + b * ((kNodeHeaderWords + 2 * sw + 3) & ~3);
+}
+__device__ __forceinline__ const uint32_t *node_at(const WaveLds &L, int b, int sw) {
+    return L.nodes + b * ((kNodeHeaderWords + 2 * sw + 3) & ~3);
+}
 
 // An LDS atomic add whose result is unused (ds_add_u64 without return), so
 // the wave does not wait for the read of a read-modify-write.
@@ -632,7 +643,7 @@ __device__ __forceinline__ bool take_one(const SearchParams &P, WaveState &W, Wa
     if (lane0(late)) { set_watchdog(P, W); return false; }
     complete_after();
     const uint32_t *src = P.pool_nodes + slot * P.node_words;
-    if (W.lane < (int)P.node_words) L.node[0][W.lane] = ld32(src + W.lane);
+    if (W.lane < (int)P.node_words) node_at(L, 0, (int)P.spi_words)[W.lane] = ld32(src + W.lane);
     complete_before();
     if (W.lane == 0) {
         st64(&P.pool_seq[slot], ticket + P.pool_cap);
@@ -744,7 +755,7 @@ template <bool kServe>
 __device__ __forceinline__ void push_staged(const SearchParams &P, WaveState &W, WaveLds &L, int n, int sw, int T,
                                             uint32_t st) {
     const int i = (int)(st & 63u), j = (int)((st >> 6) & 63u), k = (int)((st >> 12) & 63u);
-    const uint32_t *nd = L.node[st >> 24];
+    const uint32_t *nd = node_at(L, st >> 24, sw);
     const uint8_t *spi = reinterpret_cast<const uint8_t *>(nd + kNodeHeaderWords);
     const uint8_t *pi = spi + 4 * sw;
     ChildMove m;
@@ -821,7 +832,7 @@ __device__ __forceinline__ void learned_push(const SearchParams &P, WaveState &W
             const bool upper = W.lane >= kWave / 2;
             const uint32_t cst = upper ? st1 : st0;
             const int i = (int)(cst & 63u), j = (int)((cst >> 6) & 63u), k = (int)((cst >> 12) & 63u);
-            const uint32_t *nd = L.node[cst >> 24];
+            const uint32_t *nd = node_at(L, cst >> 24, sw);
             const uint8_t *cspi = reinterpret_cast<const uint8_t *>(nd + kNodeHeaderWords);
             const uint8_t *cpi = cspi + 4 * sw;
             ChildMove cm;
@@ -1163,14 +1174,14 @@ __device__ __forceinline__ int dedup_admit(const SearchParams &P, WaveLds &L, in
 template <bool kLearned, bool kServe>
 __device__ __forceinline__ void flush_stage(const SearchParams &P, WaveState &W, WaveLds &L, const ConfigDesc &C,
                                             int count, int n, int sw, int T, bool root_stage) {
-    const uint32_t cfg = L.node[0][0];   // every node of a batch has the same configuration
+    const uint32_t cfg = node_at(L, 0, sw)[0];   // every node of a batch has the same configuration
     for (int base = 0; base < count && !W.stop; base += kWave) {
         const int idx = base + W.lane;
         const bool act = idx < count;
         const uint32_t st = act ? L.stage[idx] : stage_entry(0, 1, 2, 0, 0);
         const int i = st & 63, j = (st >> 6) & 63, k = (st >> 12) & 63;
         const int ce = (int)((st >> 18) & 63u);
-        const uint32_t *nd = L.node[st >> 24];   // the child's parent
+        const uint32_t *nd = node_at(L, st >> 24, sw);   // the child's parent
         const int d = (int)(nd[1] & 0xFF);
         const uint8_t *spi = reinterpret_cast<const uint8_t *>(nd + kNodeHeaderWords);
         const uint8_t *pi = spi + 4 * sw;
@@ -1276,7 +1287,7 @@ __device__ __forceinline__ void flush_stage(const SearchParams &P, WaveState &W,
                 cm.na = c2 & 0xFFu; cm.nb = (c2 >> 8) & 0xFFu; cm.nc = c2 >> 16;
                 if (two) {
                     const int wl = W.lane & (kWave / 2 - 1);
-                    const uint32_t w = child_word(L.node[cst >> 24], wl, n, sw, ct, (int)(cst & 63u),
+                    const uint32_t w = child_word(node_at(L, cst >> 24, sw), wl, n, sw, ct, (int)(cst & 63u),
                                                   (int)((cst >> 6) & 63u), (int)((cst >> 12) & 63u),
                                                   (int)((cst >> 18) & 63u), cm);
                     uint32_t *dst = stack_slot(P, W, W.hi + (upper ? 1u : 0u));
@@ -1285,7 +1296,7 @@ __device__ __forceinline__ void flush_stage(const SearchParams &P, WaveState &W,
                     lds_add(&L.traffic[1], 2);
                     lds_add(&L.live, 2ULL);
                 } else {
-                    push_child<kServe>(P, W, L, L.node[cst >> 24], n, sw, ct, (int)(cst & 63u), (int)((cst >> 6) & 63u),
+                    push_child<kServe>(P, W, L, node_at(L, cst >> 24, sw), n, sw, ct, (int)(cst & 63u), (int)((cst >> 6) & 63u),
                                (int)((cst >> 12) & 63u), (int)((cst >> 18) & 63u), cm);
                 }
                 if (W.stop) break;
@@ -1552,7 +1563,7 @@ __device__ __forceinline__ void child_masks(const ConfigDesc &C, int d, int e, b
     }
 }
 
-// Two nodes of a batch (L.node[ba], L.node[bb], n <= 32) expanded together
+// Two nodes of a batch (nodes ba and bb of the pool, n <= 32) expanded together
 // when only delta = +2 qualifies for both: lanes 0..31 hold the symbols of
 // node ba, lanes 32..63 those of node bb, so one pass of pointer jumping
 // analyses both and one chunk loop enumerates the cycle-local children of
@@ -1569,7 +1580,7 @@ __device__ __forceinline__ int pair_local_children(WaveState &W, WaveLds &L, int
     const int l = W.lane, half = l >> 5, s = l & 31, base = l & 32;
     // cycle analysis of both nodes (build_posinfo's pointer jumping, lane
     // indices as symbols: a cycle of node bb lies in lanes 32..63)
-    const uint8_t *spi = reinterpret_cast<const uint8_t *>(L.node[half ? bb : ba] + kNodeHeaderWords);
+    const uint8_t *spi = reinterpret_cast<const uint8_t *>(node_at(L, half ? bb : ba, sw) + kNodeHeaderWords);
     const uint8_t *pi = spi + 4 * sw;
     const int nxt = (s < n) ? base + (int)spi[s] : l;
     int w = l, p = nxt;
@@ -2030,9 +2041,9 @@ __device__ __forceinline__ void search_body(const SearchParams &P) {
             }
 #pragma unroll
             for (int b = 0; b < kBatchMax; ++b)
-                if (b < want && W.lane < (int)P.node_words) L.node[b][W.lane] = v[b];
+                if (b < want && W.lane < (int)P.node_words) node_at(L, b, (int)P.spi_words)[W.lane] = v[b];
             wave_sync();
-            const bool same = W.lane < want && L.node[W.lane < kBatchMax ? W.lane : 0][0] == L.node[0][0];
+            const bool same = W.lane < want && node_at(L, W.lane < want ? W.lane : 0, (int)P.spi_words)[0] == node_at(L, 0, (int)P.spi_words)[0];
             batch = __ffsll((long long)~__ballot(same)) - 1;
             W.hi -= (uint32_t)batch;
             lds_add(&L.traffic[0], (unsigned long long)batch);
@@ -2095,7 +2106,7 @@ __device__ __forceinline__ void search_body(const SearchParams &P) {
                     w = reinterpret_cast<const uint32_t *>(C.spi0)[W.lane - kNodeHeaderWords];
                 else if (W.lane >= kNodeHeaderWords + sw && W.lane < kNodeHeaderWords + 2 * sw)
                     w = reinterpret_cast<const uint32_t *>(C.pi0)[W.lane - kNodeHeaderWords - sw];
-                L.node[0][W.lane] = w;
+                node_at(L, 0, sw)[W.lane] = w;
             }
             wave_sync();
             if (cfg_int(C.triple_count) > 0) {
@@ -2103,7 +2114,7 @@ __device__ __forceinline__ void search_body(const SearchParams &P) {
                 int scount = 0;
                 if (kLearned && W.lane == 0) L.ncarry = 0;
                 wave_sync();
-                expand_stage<kLearned, kServe>(P, W, L, C, L.node[0], 0, true, scount);
+                expand_stage<kLearned, kServe>(P, W, L, C, node_at(L, 0, sw), 0, true, scount);
                 if (scount && !W.stop) {
                     wave_sync();
                     flush_stage<kLearned, kServe>(P, W, L, C, scount, cfg_int(C.n), (int)P.spi_words, cfg_int(C.triple_count), true);
@@ -2191,9 +2202,9 @@ __device__ __forceinline__ void search_body(const SearchParams &P) {
         }
         if (!have) continue;
 
-        // ---- expand the nodes in L.node[0..batch) (one configuration): their
+        // ---- expand the nodes 0..batch of the pool (one configuration): their
         // children are staged together and deduplicated in one pass
-        const int cfg = uniform((int)L.node[0][0]);
+        const int cfg = uniform((int)node_at(L, 0, (int)P.spi_words)[0]);
         const ConfigDesc &C = cfg_copy[wib];
         {
             PHASE_T0();
@@ -2226,7 +2237,7 @@ __device__ __forceinline__ void search_body(const SearchParams &P) {
         // stack, on top.
         for (int b = batch - 1; b >= 0 && !W.stop; --b) {
             rest = b;
-            const uint32_t *nd = L.node[b];
+            const uint32_t *nd = node_at(L, b, (int)P.spi_words);
             const int d = uniform((int)(nd[1] & 0xFF));
             const int e = uniform((int)((nd[1] >> 8) & 0xFF));
             if ((cfg_bits(C.accept[d]) >> e) & 1ULL) {
@@ -2241,7 +2252,7 @@ __device__ __forceinline__ void search_body(const SearchParams &P) {
             if (!kLearned && b >= 1 && cfg_int(C.n) <= kWave / 2) {
                 // the next node too when it is expandable and both take the
                 // +2-only enumeration: one pass for the pair
-                const uint32_t *nd2 = L.node[b - 1];
+                const uint32_t *nd2 = node_at(L, b - 1, (int)P.spi_words);
                 const int d2 = uniform((int)(nd2[1] & 0xFF));
                 const int e2 = uniform((int)((nd2[1] >> 8) & 0xFF));
                 uint32_t ga, aa, gb, ab;
@@ -2262,7 +2273,7 @@ __device__ __forceinline__ void search_body(const SearchParams &P) {
                         --b;
                         rest = b;
                         if (goal_t >= 0) {
-                            report_found<kServe>(P, W, L.node[goal_b], cfg, (goal_b == b ? d2 : d) + 1, goal_t);
+                            report_found<kServe>(P, W, node_at(L, goal_b, (int)P.spi_words), cfg, (goal_b == b ? d2 : d) + 1, goal_t);
                             terminal = true;
                             break;
                         }
@@ -2290,7 +2301,7 @@ __device__ __forceinline__ void search_body(const SearchParams &P) {
             if (kLearned && !W.stop)
                 learned_push<kServe>(P, W, L, cfg_int(C.n), (int)P.spi_words, cfg_int(C.triple_count), false, 0.f, 0u, 0);
             // nodes of the batch not expanded return on top, the top one last
-            for (int b = rest - 1; b >= 0 && !W.stop; --b) push_node<kServe>(P, W, L, L.node[b]);
+            for (int b = rest - 1; b >= 0 && !W.stop; --b) push_node<kServe>(P, W, L, node_at(L, b, (int)P.spi_words));
             wave_sync();
             if (W.lane < kNumCounters) L.cnt[W.lane] += L.pend[W.lane];
             wave_sync();
@@ -2380,16 +2391,16 @@ __global__ __launch_bounds__(64) void eval_children_kernel(const uint8_t *pi, co
             const int s = 4 * lane + b;
             if (s < n) w |= (uint32_t)spi[s] << (8 * b);
         }
-        L.node[0][kNodeHeaderWords + lane] = w;
+        node_at(L, 0, sw)[kNodeHeaderWords + lane] = w;
         w = 0;
         for (int b = 0; b < 4; ++b) {
             const int s = 4 * lane + b;
             if (s < n) w |= (uint32_t)pi[s] << (8 * b);
         }
-        L.node[0][kNodeHeaderWords + sw + lane] = w;
+        node_at(L, 0, sw)[kNodeHeaderWords + sw + lane] = w;
     }
     wave_sync();
-    build_posinfo(L, L.node[0], 0, n, sw, lane);
+    build_posinfo(L, node_at(L, 0, sw), 0, n, sw, lane);
     // same enumeration and delta as the search scan, written at t = T-1-o
     int mb = -1, qb = 0;
     scan_children(colex, n, T, lane, mb, qb, [&](bool valid, int i, int j, int k, int obase) -> int {
@@ -2435,8 +2446,8 @@ __global__ __launch_bounds__(64) void eval_pair_kernel(const uint8_t *pia, const
                 if (s < n) w |= (uint32_t)spi[s] << (8 * q);
                 if (s < n) v |= (uint32_t)pi[s] << (8 * q);
             }
-            L.node[b][kNodeHeaderWords + W.lane] = w;
-            L.node[b][kNodeHeaderWords + sw + W.lane] = v;
+            node_at(L, b, sw)[kNodeHeaderWords + W.lane] = w;
+            node_at(L, b, sw)[kNodeHeaderWords + sw + W.lane] = v;
         }
     }
     wave_sync();
@@ -2446,13 +2457,13 @@ __global__ __launch_bounds__(64) void eval_pair_kernel(const uint8_t *pia, const
         if (rec == 0) {
             st = pair_local_children(W, L, 1, 0, n, sw, T, ea, eb, ga, aa, gb, ab, scount, goal_b, goal_t);
         } else {
-            uint32_t si = build_posinfo(L, L.node[1], 0, n, sw, W.lane);
-            st = cycle_local_children(W, L, L.node[1], 1, 0, si, n, sw, T, ea, ga, aa, false, scount, goal_t);
+            uint32_t si = build_posinfo(L, node_at(L, 1, sw), 0, n, sw, W.lane);
+            st = cycle_local_children(W, L, node_at(L, 1, sw), 1, 0, si, n, sw, T, ea, ga, aa, false, scount, goal_t);
             if (st == kLocalDone && goal_t >= 0) {
                 goal_b = 1;
             } else if (st == kLocalDone) {
-                si = build_posinfo(L, L.node[0], 0, n, sw, W.lane);
-                st = cycle_local_children(W, L, L.node[0], 0, 0, si, n, sw, T, eb, gb, ab, false, scount, goal_t);
+                si = build_posinfo(L, node_at(L, 0, sw), 0, n, sw, W.lane);
+                st = cycle_local_children(W, L, node_at(L, 0, sw), 0, 0, si, n, sw, T, eb, gb, ab, false, scount, goal_t);
                 if (goal_t >= 0) goal_b = 0;
             }
         }

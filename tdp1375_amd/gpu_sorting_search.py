@@ -104,7 +104,7 @@ def set_tuning(waves_per_cu: int = 0, share_interval: int = 0, share_max: int = 
 
 def set_expansion_batch(nodes: int = 0) -> None:
     """Nodes a wave expands before one dedup pass (work-sharing mode; 0 =
-    default 8)."""
+    default 12, at most 16)."""
     _lib.check(_lib.load().tdp_set_expansion_batch(int(nodes)))
 
 

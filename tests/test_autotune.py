@@ -98,7 +98,9 @@ def test_autotune_on_the_n12_batch_keeps_verdicts(gpu_lib):
     wl = W.workload_random_n12()
     A.G.init_slots(1, 1, 1 << 30, 1 << 20)
     try:
-        grid = {"share_max": [4, 8], "batch": [1, 8], "max_active": [256, 1024]}
+        # each knob: its default and one other value, so one trial per knob
+        grid = {"share_max": [4, A.DEFAULT["share_max"]], "batch": [1, A.DEFAULT["batch"]],
+                "max_active": [256, A.DEFAULT["max_active"]]}
         res = A.autotune(0, [c.pi for c in wl.cases], [c.spi for c in wl.cases], [c.even_cycles for c in wl.cases],
                          wl.min_rate, wl.max_depth, grid=grid)
         assert len(res["trials"]) == 1 + 3

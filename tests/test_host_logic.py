@@ -78,8 +78,8 @@ def test_argument_errors_without_gpu():
         T.filter_pi_permutations([0], 1, [0], 1, 0, False)
     with pytest.raises(T.TdpError, match="n exceeds PI_FILTER_MAX_N"):
         T.filter_pi_permutations([0], 1, [0], 1, 64, False)
-    for bad in (-1, 9):
-        with pytest.raises(T.TdpError, match="batch must be in 0..8"):
+    for bad in (-1, 17):
+        with pytest.raises(T.TdpError, match="batch must be in 0..16"):
             T.set_expansion_batch(bad)
     T.cancel_sort(-5)   # never throws (slot_manager.cu:100-125)
 
