@@ -1574,6 +1574,10 @@ __device__ __forceinline__ void child_masks(const ConfigDesc &C, int d, int e, b
 // children.  Returns kLocalNo (nothing done: an even cycle, too many
 // candidates for the pair or for either node) or kLocalFull (nothing done:
 // no room behind scount), else kLocalDone with goal_b / goal_t set on a goal.
+// With kPosinfo (the learned ordering) both nodes' cycle info per position
+// goes to L.posinfo[ba] and L.posinfo[bb] as build_posinfo writes it (roots
+// are lane indices there, which the features compare only for equality).
+template <bool kPosinfo>
 __device__ __forceinline__ int pair_local_children(WaveState &W, WaveLds &L, int ba, int bb, int n, int sw, int T,
                                                    int ea, int eb, uint32_t ga, uint32_t aa, uint32_t gb,
                                                    uint32_t ab, int &scount, int &goal_b, int &goal_t) {
@@ -1598,6 +1602,11 @@ __device__ __forceinline__ int pair_local_children(WaveState &W, WaveLds &L, int
     const int pos = (f == 0) ? 0 : len - f;
     const bool is_root = s < n && root == l;
     if (__ballot(is_root && (len & 1) == 0) != 0) return kLocalNo;   // even cycles: one node at a time
+    if (kPosinfo) {
+        // posinfo[p] = info(pi[p]): position p of each node in its half's lane p
+        const uint32_t at = (uint32_t)__shfl((int)cycle_info(root, pos, len), s < n ? base + (int)pi[s] : l);
+        if (s < n) L.posinfo[half ? bb : ba][s] = at;
+    }
     const bool big = is_root && len >= 3;
     const bool listed = is_root && len >= 2;
     // candidates and members over both nodes (node ba's first)
@@ -2249,7 +2258,7 @@ __device__ __forceinline__ void search_body(const SearchParams &P) {
             lds_add(&L.cnt[kCntExpansions], 1);
             lds_add(&L.cnt[kCntGenerated], (unsigned long long)cfg_int(C.triple_count));
             ++expanded;
-            if (!kLearned && b >= 1 && cfg_int(C.n) <= kWave / 2) {
+            if (b >= 1 && cfg_int(C.n) <= kWave / 2) {
                 // the next node too when it is expandable and both take the
                 // +2-only enumeration: one pass for the pair
                 const uint32_t *nd2 = node_at(L, b - 1, (int)P.spi_words);
@@ -2262,7 +2271,7 @@ __device__ __forceinline__ void search_body(const SearchParams &P) {
                     d2 < cfg_int(C.max_depth) && ((cfg_bits(C.reach[d2]) >> e2) & 1ULL)) {
                     PHASE_T0();
                     int goal_b = -1, goal_t = -1;
-                    const int r = pair_local_children(W, L, b, b - 1, cfg_int(C.n), (int)P.spi_words,
+                    const int r = pair_local_children<kLearned>(W, L, b, b - 1, cfg_int(C.n), (int)P.spi_words,
                                                       cfg_int(C.triple_count), e, e2, ga, aa, gb, ab, scount,
                                                       goal_b, goal_t);
                     PHASE_ADD(L, kPhaseScan);
@@ -2455,7 +2464,7 @@ __global__ __launch_bounds__(64) void eval_pair_kernel(const uint8_t *pia, const
         int32_t *o = out + rec * (4 + kStage);
         int scount = 0, goal_b = -1, goal_t = -1, st;
         if (rec == 0) {
-            st = pair_local_children(W, L, 1, 0, n, sw, T, ea, eb, ga, aa, gb, ab, scount, goal_b, goal_t);
+            st = pair_local_children<false>(W, L, 1, 0, n, sw, T, ea, eb, ga, aa, gb, ab, scount, goal_b, goal_t);
         } else {
             uint32_t si = build_posinfo(L, node_at(L, 1, sw), 0, n, sw, W.lane);
             st = cycle_local_children(W, L, node_at(L, 1, sw), 1, 0, si, n, sw, T, ea, ga, aa, false, scount, goal_t);
