@@ -53,9 +53,15 @@ struct WaveLds {
                                              // outside batches)
     uint32_t posinfo[kLearnedBatchMax][64];  // cycle info per position of each node of a learned batch
                                              // (row 0 only for the plain ordering)
-    uint32_t memb[64];                       // cycle members in spi order, cycles of length >= 2:
+    union {
+        struct {
+            uint32_t memb[64];               // cycle members in spi order, cycles of length >= 2:
                                              // cycle info | position in pi << 24
-    uint32_t ememb[64];                      // the members of the even cycles again, contiguous
+            uint32_t ememb[64];              // the members of the even cycles again, contiguous
+        };
+        uint32_t twin[2 * kWave];            // dedup of a chunk of staged children (the members are
+                                             // dead once the batch's children are staged)
+    };
     uint32_t stage[kStage];
     uint8_t rowsrc[64];                      // learned ordering: lane of each compacted admitted child
     float carry_score[kWave];                // learned ordering: the best-scored admitted children of
@@ -960,6 +966,30 @@ __device__ __forceinline__ int dedup_admit(const SearchParams &P, WaveLds &L, in
     const int nw2 = (nw + 1) >> 1;
     const unsigned long long epoch = gen;
     int res = active ? kDedupPending : kDedupDominated;
+    {
+        // Equal children in one chunk: sibling nodes of a batch reach common
+        // grandchildren by commuting moves.  Of the lanes holding one key,
+        // the one of least (depth, lane) looks it up and the others are
+        // dominated by it here (its lookup leaves the key in the table at a
+        // depth no greater than theirs, or finds it there, or admits it).
+        // The lanes meet in a slot of L.twin chosen by the hash; a lane whose
+        // slot another key holds is left to the table.
+        const uint32_t me = ((uint32_t)depth << 6) | (uint32_t)lane;
+        const int slot = (int)(h & (2 * kWave - 1));
+        L.twin[lane] = ~0u;
+        L.twin[lane + kWave] = ~0u;
+        wave_sync();
+        if (active) __hip_atomic_fetch_min(&L.twin[slot], me, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+        wave_sync();
+        const uint32_t held = L.twin[slot];
+        if (active && held != me) {
+            const int o = (int)(held & 63u);
+            bool eq = true;
+            for (int q = 0; q < nw; ++q) eq &= L.ckey[q][o] == L.ckey[q][lane];
+            if (eq) res = kDedupDominated;
+        }
+        wave_sync();
+    }
     int rounds = 0;
     bool slow = false;
     // the victim for this child: a free entry (serving: a zero tag; batch: an
