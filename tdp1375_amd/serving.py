@@ -149,6 +149,10 @@ def main(argv=None):
     ap.add_argument("--duration", type=float, default=4.0)
     ap.add_argument("--ordering", choices=("plain", "learned"), default="plain")
     ap.add_argument("--model", default=None)
+    ap.add_argument("--share-interval", type=int, default=None,
+                    help="expansions between hand-off checks (default: 2 for the learned ordering, whose "
+                         "requests settle in a few expansions and leave hard ones to spread, else the "
+                         "library default)")
     ap.add_argument("--pool", default=os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
                                                    "tests", "golden", "serving_pool_n9-48-d11.json"),
                     help="labelled request pool (indices into pool_cases and their verdicts)")
@@ -161,6 +165,10 @@ def main(argv=None):
     if args.ordering == "learned":
         from . import train_ordering
         train_ordering.load(args.model or os.path.join(os.path.dirname(__file__), "data", "ordering_mlp_d11.npz"))
+    share_interval = args.share_interval if args.share_interval is not None else (2 if args.ordering == "learned" else 0)
+    if share_interval:
+        from . import gpu_sorting_search
+        gpu_sorting_search.set_tuning(share_interval=share_interval)
     with DeviceServer(learned=args.ordering == "learned", max_n=48) as server:
         for r in [float(x) for x in args.rates.split(",")]:
             out = poisson_run(server, pool, r, args.duration, rate, doc["max_depth"], seed=int(r))
