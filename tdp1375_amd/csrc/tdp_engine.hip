@@ -267,7 +267,8 @@ const TripleTable &triples() {
 
 struct DeviceShared {
 
-    int waves_max = 0;   // resident waves of search_kernel
+    int waves_max = 0;           // resident waves of search_kernel
+    int waves_max_learned = 0;   // resident waves of search_learned_kernel
     int cus = 0;
 };
 std::mutex g_dev_mutex;
@@ -285,6 +286,9 @@ DeviceShared &device_shared(int device) {
         int blocks = 0;
         HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, search_kernel, 256, 0));
         d->waves_max = std::max(1, blocks) * d->cus * 4;
+        blocks = 0;
+        HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, search_learned_kernel, 256, 0));
+        d->waves_max_learned = std::max(1, blocks) * d->cus * 4;
         g_dev[device] = std::move(d);
     }
     return *g_dev[device];
@@ -686,14 +690,17 @@ void run_launch(Slot &s, const std::vector<CaseInput> &cases, uint32_t flags, bo
 
     int waves;
     bool donate;
+    // every wave of a persistent launch must be resident at once: the grid
+    // is the launched kernel's occupancy limit at most
+    const int resident_waves = (flags & TDP_FLAG_LEARNED) ? dev.waves_max_learned : dev.waves_max;
     if (flags & TDP_FLAG_SINGLE_WAVE) {
         waves = 1;
         donate = false;
     } else if (flags & TDP_FLAG_DETERMINISTIC) {
-        waves = std::min(count, dev.waves_max);
+        waves = std::min(count, resident_waves);
         donate = false;
     } else {
-        waves = dev.waves_max;
+        waves = resident_waves;
         donate = true;
     }
     // queue memory (queueBytesBudget, sorting_search.cu:655-658): the

@@ -2377,22 +2377,27 @@ __device__ __forceinline__ void search_body(const SearchParams &P) {
 
 // Everything is inlined into the kernels: a call would pass the by-value
 // parameter block through scratch memory and LDS references as 64-bit flat
-// pointers.  Three waves per SIMD (<= 168 VGPRs) costs a few spilled
-// registers; four would spill heavily.  The learned-ordering variant is a
-// kernel of its own so that the plain search carries no MFMA code.
+// pointers.  Three waves per SIMD (<= 168 VGPRs) costs the plain kernels a
+// few spilled registers; four would spill heavily.  The learned-ordering
+// variant is a kernel of its own so that the plain search carries no MFMA
+// code; it runs at two waves per SIMD (<= 256 VGPRs), where its scoring
+// state (MLP fragments, features, the sort's keys) needs no spills.
 #ifndef TDP_WAVES_PER_EU
 #define TDP_WAVES_PER_EU 3
+#endif
+#ifndef TDP_LEARNED_WAVES_PER_EU
+#define TDP_LEARNED_WAVES_PER_EU 2
 #endif
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TDP_WAVES_PER_EU))) void search_kernel(SearchParams P) {
     search_body<false, false>(P);
 }
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TDP_WAVES_PER_EU))) void search_learned_kernel(SearchParams P) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TDP_LEARNED_WAVES_PER_EU))) void search_learned_kernel(SearchParams P) {
     search_body<true, false>(P);
 }
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TDP_WAVES_PER_EU))) void serve_kernel(SearchParams P) {
     search_body<false, true>(P);
 }
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TDP_WAVES_PER_EU))) void serve_learned_kernel(SearchParams P) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TDP_LEARNED_WAVES_PER_EU))) void serve_learned_kernel(SearchParams P) {
     search_body<true, true>(P);
 }
 
