@@ -112,6 +112,21 @@ def test_dataset_groups_hold_one_taken_move():
 
 
 # ---------------------------------------------------------------- GPU
+@pytest.mark.parametrize("argv, message", [
+    (["--seeds", "3,11"], "seed 3"),
+    (["--validation-seed", "3"], "seed 3"),
+    (["--validation-seed", "12"], "validation seed"),
+    (["--candidates", "0"], "--candidates"),
+])
+def test_training_refuses_benchmark_and_overlapping_seeds(argv, message):
+    # the benchmark batch (seed 3) never trains or picks a network, and the
+    # validation batch is not a training batch; refused before any GPU call
+    from tdp1375_amd import train_ordering
+    with pytest.raises(SystemExit) as ex:
+        train_ordering.main(["--out", "/nonexistent/model.npz"] + argv)
+    assert message in str(ex.value)
+
+
 @pytest.mark.gpu
 def test_gpu_features_equal_numpy(gpu_lib):
     rng = np.random.default_rng(17)
