@@ -36,7 +36,7 @@ GRID = {
     "share_interval": [8, 16, 32],
     "share_max": [4, 8, 16],
     "batch": [4, 8, 12, 16],
-    "max_active": [512, 1024, 2048],
+    "max_active": [512, 1024, 1536, 2048, 3072],
 }
 # The learned ordering settles most configurations within a few expansions
 # and leaves a tail of hard ones, each expansion costing more (the scoring):
