@@ -199,7 +199,7 @@ TDP_API int tdp_filter(const uint8_t *cycle, int cycle_len, const uint8_t *tripl
  * stream) and returns: out_rows_dev receives the first min(count,
  * out_rows_cap) passing rows (n bytes each, rank order), *out_total_dev the
  * uncapped count once the stream reaches that point.  Working memory is a
- * fixed 11.2 MiB device array inside the library (rank passes of 2^26, the
+ * fixed 8.5 MiB device array inside the library (passes of 2^21 groups of ranks, the
  * cycle's 2-move table), one per GPU for the life of the process, so any
  * valid input fits it; calls that overlap on one GPU (another thread or
  * stream) race on that array and must be serialised by the caller. */

@@ -1479,6 +1479,8 @@ FilterParams filter_params(const uint8_t *cycle, int L, const uint8_t *triple, i
     std::memcpy(F.triple, triple, (size_t)tl);
     F.fact[0] = 1;
     for (int i = 1; i < 24; ++i) F.fact[i] = F.fact[i - 1] * (uint64_t)(i < 21 ? i : 1);
+    const uint64_t group = (uint64_t)filter_group_size(n);
+    for (int i = 0; i < 24; ++i) F.gfact[i] = i >= filter_tail(n) ? F.fact[i] / group : 0;
     F.rank32 = F.fact[n - 1] < (1ULL << 32) ? 1 : 0;
     return F;
 }
@@ -1523,11 +1525,19 @@ void filter_impl(const uint8_t *cycle, int L, const uint8_t *triple, int tl, int
         g_filter_ws[device] = std::move(w);
     }
     FilterWorkspace &W = *g_filter_ws[device];
-    constexpr uint64_t kChunk = 1ULL << 32;   // ranks per launch chunk
+    // ranks per launch chunk: 2^27 groups (a multiple of the group size, so
+    // every chunk but the last holds whole groups; the total is one too)
+    const uint64_t group = (uint64_t)filter_group_size(n);
+    const uint64_t kChunk = group << 27;
     const uint64_t chunk = std::min<uint64_t>(total, kChunk);
-    const size_t blocks = (size_t)((chunk + 255) / 256);
+    const uint64_t chunk_groups = (chunk + group - 1) / group;
+    F.split_log2 = filter_split_log2(chunk_groups);
+    const size_t blocks = (size_t)(((chunk_groups << F.split_log2) + 255) / 256);
+    if (W.mask_words < chunk_groups) {
+        W.mask = make_dev((size_t)chunk_groups * 4, "filter.mask");
+        W.mask_words = (size_t)chunk_groups;
+    }
     if (W.blocks < blocks) {
-        W.mask = make_dev(blocks * 4 * 8, "filter.mask");
         W.counts = make_dev(blocks * 4, "filter.counts");
         W.offsets = make_dev(blocks * 8, "filter.offsets");
         W.blocks = blocks;
@@ -1545,7 +1555,7 @@ void filter_impl(const uint8_t *cycle, int L, const uint8_t *triple, int tl, int
     if (!moves.empty())
         HIP_CHECK(hipMemcpyAsync(W.moves.get(), moves.data(), moves.size() * 4, hipMemcpyHostToDevice, W.stream));
     F.moves = as<uint32_t>(W.moves);
-    F.mask = as<unsigned long long>(W.mask);
+    F.mask = as<uint32_t>(W.mask);
     F.block_counts = as<uint32_t>(W.counts);
     F.out = as<uint8_t>(W.out);
     F.out_cap = cap_rows;
@@ -1554,15 +1564,22 @@ void filter_impl(const uint8_t *cycle, int L, const uint8_t *triple, int tl, int
     for (uint64_t off = 0; off < total; off += chunk) {
         F.rank_offset = off;
         F.rank_end = std::min(total, off + chunk);
-        const uint64_t nb = (F.rank_end - off + 255) / 256;
-        hipLaunchKernelGGL(filter_mask_kernel, dim3((unsigned)nb), dim3(256), 0, W.stream, F);
+        const uint64_t nb = ((((F.rank_end - off + group - 1) / group) << F.split_log2) + 255) / 256;
+        if (filter_nibbles(n))
+            hipLaunchKernelGGL(filter_mask_kernel, dim3((unsigned)nb), dim3(256), 0, W.stream, F);
+        else
+            hipLaunchKernelGGL(filter_mask_any_kernel, dim3((unsigned)nb), dim3(256), 0, W.stream, F);
         HIP_CHECK(hipGetLastError());
         hipLaunchKernelGGL(filter_scan_kernel, dim3(1), dim3(1024), 0, W.stream, as<uint32_t>(W.counts), nb,
                            as<uint64_t>(W.offsets), as<uint64_t>(W.total));
         HIP_CHECK(hipGetLastError());
         if (cap_rows > 0) {
-            hipLaunchKernelGGL(filter_write_kernel, dim3((unsigned)nb), dim3(256), 0, W.stream, F,
-                               (const uint64_t *)as<uint64_t>(W.offsets));
+            if (filter_nibbles(n))
+                hipLaunchKernelGGL(filter_write_kernel, dim3((unsigned)nb), dim3(256), 0, W.stream, F,
+                                   (const uint64_t *)as<uint64_t>(W.offsets));
+            else
+                hipLaunchKernelGGL(filter_write_any_kernel, dim3((unsigned)nb), dim3(256), 0, W.stream, F,
+                                   (const uint64_t *)as<uint64_t>(W.offsets));
             HIP_CHECK(hipGetLastError());
         }
     }
@@ -1583,16 +1600,16 @@ void filter_impl(const uint8_t *cycle, int L, const uint8_t *triple, int tl, int
 // ---------------------------------------------------------------- allocation-free filter
 // tdp_filter_noalloc runs the filter's passes on the caller's stream with all
 // of its working memory in `scratch`, a file-scope device array: rank chunks
-// of 2^26 (8 MiB of masks, 1 MiB of block counts, 2 MiB of offsets) and the
-// 2-move table (at most C(64,3) entries) with its count.  Each device holds
+// of 2^21 groups (8 MiB of masks, 128 KiB of block counts, 256 KiB of offsets)
+// and the 2-move table (at most C(64,3) entries) with its count.  Each device holds
 // its own copy of the array from library load to process end; host code
 // reaches it only through the address hipGetSymbolAddress returns for the
 // current device, resolved once per device on the first call there.
-constexpr uint64_t kNoAllocChunk = 1ULL << 26;                           // ranks per pass
-constexpr size_t kNoAllocBlocks = (size_t)(kNoAllocChunk / 256);         // mask kernel blocks
+constexpr uint64_t kNoAllocGroups = 1ULL << 21;                          // rank groups per pass
+constexpr size_t kNoAllocBlocks = (size_t)(kNoAllocGroups * kFilterMaxThreadsPerGroup / 256);   // mask kernel blocks
 constexpr size_t kNoAllocMaxMoves = 64 * 63 * 62 / 6;                    // C(64,3)
 constexpr size_t kScratchMaskOff = 0;
-constexpr size_t kScratchCountsOff = kScratchMaskOff + kNoAllocBlocks * 4 * 8;
+constexpr size_t kScratchCountsOff = kScratchMaskOff + kNoAllocGroups * 4;
 constexpr size_t kScratchOffsetsOff = kScratchCountsOff + kNoAllocBlocks * 4;
 constexpr size_t kScratchMovesOff = kScratchOffsetsOff + kNoAllocBlocks * 8;
 constexpr size_t kScratchMoveCountOff = kScratchMovesOff + ((kNoAllocMaxMoves * 4 + 255) & ~(size_t)255);
@@ -1621,7 +1638,7 @@ void filter_noalloc_impl(const uint8_t *cycle, int L, const uint8_t *triple, int
     if (!out_total) throw std::invalid_argument("null argument");
     if (out_cap > 0 && !out_rows) throw std::invalid_argument("null argument");
     // every working buffer of this call fits the scratch array by
-    // construction: one pass takes at most kNoAllocChunk ranks and a cycle
+    // construction: one pass takes at most kNoAllocGroups rank groups and a cycle
     // of at most 64 symbols has at most C(64,3) 2-move candidates
     const size_t max_moves = L >= 3 ? (size_t)L * (L - 1) * (L - 2) / 6 : 0;
     if (max_moves > kNoAllocMaxMoves) throw std::invalid_argument("cycle longer than the scratch array allows");
@@ -1633,7 +1650,7 @@ void filter_noalloc_impl(const uint8_t *cycle, int L, const uint8_t *triple, int
     uint32_t *move_count = reinterpret_cast<uint32_t *>(base + kScratchMoveCountOff);
     F.moves = moves;
     F.move_count_dev = move_count;
-    F.mask = reinterpret_cast<unsigned long long *>(base + kScratchMaskOff);
+    F.mask = reinterpret_cast<uint32_t *>(base + kScratchMaskOff);
     F.block_counts = reinterpret_cast<uint32_t *>(base + kScratchCountsOff);
     uint64_t *offsets = reinterpret_cast<uint64_t *>(base + kScratchOffsetsOff);
     F.out = out_rows;
@@ -1643,18 +1660,28 @@ void filter_noalloc_impl(const uint8_t *cycle, int L, const uint8_t *triple, int
     // accumulate (no memset: the sequence may be captured into a graph)
     hipLaunchKernelGGL(filter_moves_kernel, dim3(1), dim3(64), 0, stream, F, moves, move_count, out_total);
     HIP_CHECK(hipGetLastError());
-    for (uint64_t off = 0; off < total; off += kNoAllocChunk) {
+    const uint64_t group = (uint64_t)filter_group_size(n);
+    const uint64_t chunk = group * kNoAllocGroups;   // whole groups per pass
+    F.split_log2 = filter_split_log2(std::min<uint64_t>(kNoAllocGroups, (total + group - 1) / group));
+    for (uint64_t off = 0; off < total; off += chunk) {
         F.rank_offset = off;
-        F.rank_end = std::min(total, off + kNoAllocChunk);
-        const uint64_t nb = (F.rank_end - off + 255) / 256;
-        hipLaunchKernelGGL(filter_mask_kernel, dim3((unsigned)nb), dim3(256), 0, stream, F);
+        F.rank_end = std::min(total, off + chunk);
+        const uint64_t nb = ((((F.rank_end - off + group - 1) / group) << F.split_log2) + 255) / 256;
+        if (filter_nibbles(n))
+            hipLaunchKernelGGL(filter_mask_kernel, dim3((unsigned)nb), dim3(256), 0, stream, F);
+        else
+            hipLaunchKernelGGL(filter_mask_any_kernel, dim3((unsigned)nb), dim3(256), 0, stream, F);
         HIP_CHECK(hipGetLastError());
         hipLaunchKernelGGL(filter_scan_kernel, dim3(1), dim3(1024), 0, stream, F.block_counts, nb, offsets,
                            out_total);
         HIP_CHECK(hipGetLastError());
         if (out_cap > 0) {
-            hipLaunchKernelGGL(filter_write_kernel, dim3((unsigned)nb), dim3(256), 0, stream, F,
-                               (const uint64_t *)offsets);
+            if (filter_nibbles(n))
+                hipLaunchKernelGGL(filter_write_kernel, dim3((unsigned)nb), dim3(256), 0, stream, F,
+                                   (const uint64_t *)offsets);
+            else
+                hipLaunchKernelGGL(filter_write_any_kernel, dim3((unsigned)nb), dim3(256), 0, stream, F,
+                                   (const uint64_t *)offsets);
             HIP_CHECK(hipGetLastError());
         }
     }
