@@ -12,7 +12,7 @@ CYCLE = FILTER_ORIENTED_5_CYCLE["cycle"]
 TRIPLE = FILTER_ORIENTED_5_CYCLE["triple"]
 
 
-@pytest.mark.parametrize("n", [1, 2, 5, 6, 8, 10])
+@pytest.mark.parametrize("n", [1, 2, 3, 4, 5, 6, 7, 8, 9, 10])
 @pytest.mark.parametrize("skip", [False, True])
 def test_filter_small_exact(gpu_lib, n, skip):
     cyc = [c for c in CYCLE if c < n] if n < 5 else CYCLE
