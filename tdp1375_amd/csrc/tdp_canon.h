@@ -15,8 +15,9 @@ constexpr int kMaxCanonN = 64;
 // and pi (pis[c * 64 + i] = symbol at position i) of ns[c] symbols; writes the
 // canonical signature (64 floats per configuration), its hash, the index of
 // the equivalent signature it came from (2 * rotation + mirror) and the
-// canonical spi (one-line, pi the identity).  Device pointers.
-void launch_canon(const uint8_t *spis, const uint8_t *pis, const int32_t *ns, int count, float *out_sig,
+// canonical spi (one-line, pi the identity).  Device pointers; max_n (host)
+// bounds every ns[c] and sizes the kernel's LDS.
+void launch_canon(const uint8_t *spis, const uint8_t *pis, const int32_t *ns, int count, int max_n, float *out_sig,
                   int32_t *out_hash, int32_t *out_index, uint8_t *out_spi, hipStream_t stream);
 
 }  // namespace tdp

@@ -9,7 +9,9 @@
 //
 // One wave per configuration (n <= 64): lanes are symbols for the cycle
 // analysis and positions for the signature; the 2n signatures are kept in
-// LDS and every lane hashes its own candidates.
+// LDS (dynamic, sized by the batch's largest n: 10 KiB at n = 36 where a
+// fixed n = 64 array took 32 KiB and held a CU to four waves) and every lane
+// hashes its own candidates.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -23,7 +25,6 @@ constexpr int kW = 64;
 constexpr int kMaxCand = 2 * kMaxCanonN;
 
 struct CanonLds {
-    float sig[kMaxCand][kMaxCanonN];   // candidate signatures
     uint32_t crc_table[256];
     uint32_t root[kW], cpos[kW], clen[kW];   // cycle of each symbol: min symbol, steps from it, length
     uint32_t cnt[kW];                  // per cycle root: ascents along the cycle
@@ -82,10 +83,10 @@ __device__ __forceinline__ void analyse(CanonLds &L, int n, int lane) {
     sync_wave();
 }
 
-// Signature of (L.perm, pi rotated by r) into L.sig[c]; L.pos holds the
-// positions in the rotated pi (lane = symbol) and `at` the symbol at this
-// lane's position.
-__device__ __forceinline__ void sign(CanonLds &L, int n, int lane, int at, int c) {
+// Signature of (L.perm, pi rotated by r) into candidate c of `sig` (rows of
+// `stride` floats); L.pos holds the positions in the rotated pi (lane =
+// symbol) and `at` the symbol at this lane's position.
+__device__ __forceinline__ void sign(CanonLds &L, float *sig, int stride, int n, int lane, int at, int c) {
     if (lane < n) {
         L.cnt[lane] = 0;
         L.first[lane] = 0xFFFFFFFFu;
@@ -115,15 +116,18 @@ __device__ __forceinline__ void sign(CanonLds &L, int n, int lane, int at, int c
             const uint32_t k = (L.cpos[at] + len - L.minp[rt]) % len + 1u;
             v = label + (float)k / 100.0f;
         }
-        L.sig[c][lane] = v;
+        sig[c * stride + lane] = v;
     }
     sync_wave();
 }
 
+// stride: the batch's largest n; the launch gives 2 * stride * stride floats
+// of dynamic LDS for the candidates.
 __global__ __launch_bounds__(64) void canon_kernel(const uint8_t *spis, const uint8_t *pis, const int32_t *ns,
-                                                    int count, float *out_sig, int32_t *out_hash,
+                                                    int count, int stride, float *out_sig, int32_t *out_hash,
                                                     int32_t *out_index, uint8_t *out_spi) {
     __shared__ CanonLds L;
+    extern __shared__ float sig[];
     const int lane = threadIdx.x;
     // CRC-32 (reflected polynomial 0xEDB88320), the table once per block
     for (int b = lane; b < 256; b += kW) {
@@ -151,7 +155,7 @@ __global__ __launch_bounds__(64) void canon_kernel(const uint8_t *spis, const ui
         sync_wave();
         if (r == 0) analyse(L, n, lane);   // the same cycles for every rotation
         else sync_wave();
-        sign(L, n, lane, at, 2 * r);
+        sign(L, sig, stride, n, lane, at, 2 * r);
         // its mirror: spi conjugated by tau = prod (pi_r[i] pi_r[n-1-i]),
         // then inverted: x -> tau(spi^-1(tau(x)))
         if (lane < n) {
@@ -161,7 +165,7 @@ __global__ __launch_bounds__(64) void canon_kernel(const uint8_t *spis, const ui
         }
         sync_wave();
         analyse(L, n, lane);
-        sign(L, n, lane, at, 2 * r + 1);
+        sign(L, sig, stride, n, lane, at, 2 * r + 1);
         // the cycles of spi again for the next rotation
         if (lane < n) L.perm[lane] = spi[lane];
         sync_wave();
@@ -171,7 +175,7 @@ __global__ __launch_bounds__(64) void canon_kernel(const uint8_t *spis, const ui
     for (int c = lane; c < nc; c += kW) {
         uint32_t crc = 0xFFFFFFFFu;
         for (int i = 0; i < n; ++i) {
-            const uint32_t b = __float_as_uint(L.sig[c][i]);
+            const uint32_t b = __float_as_uint(sig[c * stride + i]);
 #pragma unroll
             for (int s = 24; s >= 0; s -= 8) crc = L.crc_table[(crc ^ (b >> s)) & 0xFFu] ^ (crc >> 8);
         }
@@ -187,7 +191,7 @@ __global__ __launch_bounds__(64) void canon_kernel(const uint8_t *spis, const ui
             bool less = h < hb;
             if (h == hb) {
                 for (int i = 0; i < n; ++i) {
-                    const float a = L.sig[c][i], bb = L.sig[best][i];
+                    const float a = sig[c * stride + i], bb = sig[best * stride + i];
                     if (a != bb) {
                         less = a < bb;
                         break;
@@ -198,7 +202,7 @@ __global__ __launch_bounds__(64) void canon_kernel(const uint8_t *spis, const ui
         }
     }
     best = __shfl(best, 0);
-    if (lane < n) out_sig[(size_t)cfg * kMaxCanonN + lane] = L.sig[best][lane];
+    if (lane < n) out_sig[(size_t)cfg * kMaxCanonN + lane] = sig[best * stride + lane];
     if (lane == 0) {
         out_hash[cfg] = L.hash[best];
         out_index[cfg] = best;
@@ -206,7 +210,7 @@ __global__ __launch_bounds__(64) void canon_kernel(const uint8_t *spis, const ui
     // the configuration of that signature (pi the identity): an oriented
     // cycle in rank order, any other by decreasing position
     if (lane < n) {
-        const float v = L.sig[best][lane];
+        const float v = sig[best * stride + lane];
         const uint32_t lab = (uint32_t)v;
         const bool oriented = v != (float)lab;
         L.lab_of[lane] = lab;
@@ -232,10 +236,12 @@ __global__ __launch_bounds__(64) void canon_kernel(const uint8_t *spis, const ui
 
 }  // namespace
 
-void launch_canon(const uint8_t *spis, const uint8_t *pis, const int32_t *ns, int count, float *out_sig,
+void launch_canon(const uint8_t *spis, const uint8_t *pis, const int32_t *ns, int count, int max_n, float *out_sig,
                   int32_t *out_hash, int32_t *out_index, uint8_t *out_spi, hipStream_t stream) {
-    hipLaunchKernelGGL(canon_kernel, dim3((unsigned)count), dim3(kW), 0, stream, spis, pis, ns, count, out_sig,
-                       out_hash, out_index, out_spi);
+    const int stride = max_n < 1 ? 1 : max_n;
+    const size_t lds = (size_t)2 * stride * stride * sizeof(float);
+    hipLaunchKernelGGL(canon_kernel, dim3((unsigned)count), dim3(kW), lds, stream, spis, pis, ns, count, stride,
+                       out_sig, out_hash, out_index, out_spi);
 }
 
 }  // namespace tdp

@@ -2040,9 +2040,11 @@ int tdp_canonical_batch(int count, const uint8_t *spis, const uint8_t *pis, cons
             throw std::invalid_argument("null argument");
         // Configuration.java takes spi without fixed points (every symbol in a
         // cycle of the string form) and pi a cycle over the same symbols
+        int max_n = 1;
         for (int c = 0; c < count; ++c) {
             const int n = ns[c];
             if (n < 1 || n > kMaxCanonN) throw std::invalid_argument("configuration size must be in 1..64");
+            max_n = std::max(max_n, n);
             const uint8_t *spi = spis + (size_t)c * kMaxCanonN, *pi = pis + (size_t)c * kMaxCanonN;
             uint64_t seen_s = 0, seen_p = 0;
             for (int x = 0; x < n; ++x) {
@@ -2064,7 +2066,7 @@ int tdp_canonical_batch(int count, const uint8_t *spis, const uint8_t *pis, cons
         HIP_CHECK(hipMemcpy(d_n.get(), ns, (size_t)count * 4, hipMemcpyHostToDevice));
         HIP_CHECK(hipMemset(d_sig.get(), 0, rows * 4));
         HIP_CHECK(hipMemset(d_out.get(), 0, rows));
-        launch_canon(as<uint8_t>(d_spi), as<uint8_t>(d_pi), as<int32_t>(d_n), count, as<float>(d_sig),
+        launch_canon(as<uint8_t>(d_spi), as<uint8_t>(d_pi), as<int32_t>(d_n), count, max_n, as<float>(d_sig),
                      as<int32_t>(d_hash), as<int32_t>(d_idx), as<uint8_t>(d_out), nullptr);
         HIP_CHECK(hipGetLastError());
         HIP_CHECK(hipDeviceSynchronize());
