@@ -27,8 +27,10 @@ form has no sorting is extended into the next level.
 from __future__ import annotations
 
 import math
+import os
 import time
 from dataclasses import dataclass, field
+from multiprocessing import Pool
 from typing import Callable, Dict, List, Optional, Sequence, Set, Tuple
 
 import numpy as np
@@ -584,9 +586,15 @@ def _keys(rows: np.ndarray) -> List[bytes]:
     return np.ascontiguousarray(rows).view(np.dtype((np.void, ROW_WIDTH))).ravel().tolist()
 
 
+def _extend_chunk(args) -> List[Tuple[np.ndarray, np.ndarray]]:
+    extend_rows, rows = args
+    return [extend_rows(r) for r in rows]
+
+
 def explore(roots: Sequence[Cycles], extend_rows: Callable[[Sequence[int]], Tuple[np.ndarray, np.ndarray]],
             max_levels: int, max_ratio: float = 1.375, max_depth: int = 11, slot: int = 0,
-            max_configurations: int = 1 << 21, log: Optional[Callable[[str], None]] = None) -> Exploration:
+            max_configurations: int = 1 << 21, log: Optional[Callable[[str], None]] = None,
+            workers: int = 1) -> Exploration:
     """The SortOrExtend tree from roots, level by level, to max_levels levels,
     with extend_rows (dfs_extension_rows or combination_extension_rows).
 
@@ -596,7 +604,10 @@ def explore(roots: Sequence[Cycles], extend_rows: Callable[[Sequence[int]], Tupl
     of the reference); every configuration whose canonical form has no
     sorting is extended.  A level with more than max_configurations distinct
     configurations stops the exploration (truncated=True).  verdicts and
-    sortings are keyed by the canonical one-line spi as a tuple."""
+    sortings are keyed by the canonical one-line spi as a tuple.  With
+    workers > 1 the configurations of a level are extended on that many host
+    processes (extend_rows must be a module-level function), in chunks whose
+    results are joined in the same order as one process would."""
     from . import canonical as K
     from . import proof_driver as D
 
@@ -607,65 +618,84 @@ def explore(roots: Sequence[Cycles], extend_rows: Callable[[Sequence[int]], Tupl
     levels: List[Level] = []
     rows, ns = _pad([np.asarray([one_line_of(r)], np.int64) for r in roots])
     truncated = False
-    for depth in range(max_levels):
-        t0 = time.perf_counter()
-        view = np.ascontiguousarray(rows).view(np.dtype((np.void, ROW_WIDTH))).ravel()
-        keys = view.tolist()
-        first = np.sort(np.unique(view, return_index=True)[1])
-        fresh = [int(i) for i in first if keys[i] not in seen]
-        if not fresh:
-            break
-        if len(fresh) > max_configurations:
-            truncated = True
-            break
-        seen.update(keys[i] for i in fresh)
-        rows, ns = rows[fresh], ns[fresh]
-        t1 = time.perf_counter()
-        _, _, _, canon = K.canonical_rows(rows, ns)
-        canon[np.arange(ROW_WIDTH)[None, :] >= ns[:, None]] = PAD
-        ckeys = _keys(canon)
-        t2 = time.perf_counter()
-        todo = sorted({k for k in ckeys if k not in verdicts})
-        found = 0
-        for lo in range(0, len(todo), SEARCH_BATCH_MAX):
-            part = todo[lo:lo + SEARCH_BATCH_MAX]
-            R = np.frombuffer(b"".join(part), np.uint8).reshape(len(part), ROW_WIDTH)
-            N = (R != PAD).sum(axis=1).astype(np.int32)
-            e0 = D.odd_cycle_counts(R, N)
-            three = (N - e0) // 2
-            res = D.sort_rows(slot, R, N, max_ratio, max_depth, e0=e0, norms=three)
-            for q, k in enumerate(part):
-                verdicts[k] = bool(res.found[q])
-                norms[k] = int(three[q])
-                if res.found[q]:
-                    sortings[tuple(R[q, :N[q]].tolist())] = res.moves[q]
-                    found += 1
-        t3 = time.perf_counter()
-        bad = [i for i, k in enumerate(ckeys) if not verdicts[k]]
-        last = depth + 1 == max_levels
-        parts_r, parts_n = [], []
-        tick = time.perf_counter()
-        if not last:
-            for t, i in enumerate(bad):
-                r_, n_ = extend_rows(rows[i, :ns[i]])
-                parts_r.append(r_)
-                parts_n.append(n_)
-                if log and time.perf_counter() - tick > 30:
-                    tick = time.perf_counter()
-                    log(f"level {depth}: extended {t + 1} of {len(bad)} configurations")
-        t4 = time.perf_counter()
-        lv = Level(depth, len(fresh), len(todo), found, len(bad), max(norms[k] for k in set(ckeys)),
-                   max((norms[ckeys[i]] for i in bad), default=-1),
-                   {"dedup": t1 - t0, "canonical": t2 - t1, "search": t3 - t2, "extend": t4 - t3})
-        levels.append(lv)
-        if log:
-            log(f"level {depth}: {lv.configurations} configurations, {lv.new_canonical} new canonical, "
-                f"{lv.found} found, {lv.extended} without a sorting (3-norm <= {lv.max_bad_three_norm}), "
-                f"3-norm <= {lv.max_three_norm}, "
-                + ", ".join(f"{k} {v:.2f}s" for k, v in lv.seconds.items()))
-        if last or not parts_r:
-            break
-        rows, ns = np.concatenate(parts_r), np.concatenate(parts_n)
+    pool = None
+    try:
+        for depth in range(max_levels):
+            t0 = time.perf_counter()
+            view = np.ascontiguousarray(rows).view(np.dtype((np.void, ROW_WIDTH))).ravel()
+            keys = view.tolist()
+            first = np.sort(np.unique(view, return_index=True)[1])
+            fresh = [int(i) for i in first if keys[i] not in seen]
+            if not fresh:
+                break
+            if len(fresh) > max_configurations:
+                truncated = True
+                break
+            seen.update(keys[i] for i in fresh)
+            rows, ns = rows[fresh], ns[fresh]
+            t1 = time.perf_counter()
+            _, _, _, canon = K.canonical_rows(rows, ns)
+            canon[np.arange(ROW_WIDTH)[None, :] >= ns[:, None]] = PAD
+            ckeys = _keys(canon)
+            t2 = time.perf_counter()
+            todo = sorted({k for k in ckeys if k not in verdicts})
+            found = 0
+            for lo in range(0, len(todo), SEARCH_BATCH_MAX):
+                part = todo[lo:lo + SEARCH_BATCH_MAX]
+                R = np.frombuffer(b"".join(part), np.uint8).reshape(len(part), ROW_WIDTH)
+                N = (R != PAD).sum(axis=1).astype(np.int32)
+                e0 = D.odd_cycle_counts(R, N)
+                three = (N - e0) // 2
+                res = D.sort_rows(slot, R, N, max_ratio, max_depth, e0=e0, norms=three)
+                for q, k in enumerate(part):
+                    verdicts[k] = bool(res.found[q])
+                    norms[k] = int(three[q])
+                    if res.found[q]:
+                        sortings[tuple(R[q, :N[q]].tolist())] = res.moves[q]
+                        found += 1
+            t3 = time.perf_counter()
+            bad = [i for i, k in enumerate(ckeys) if not verdicts[k]]
+            last = depth + 1 == max_levels
+            parts_r, parts_n = [], []
+            tick = time.perf_counter()
+            if not last and workers > 1 and len(bad) >= 2 * workers:
+                if pool is None:
+                    pool = Pool(workers)
+                todo_rows = [rows[i, :ns[i]] for i in bad]
+                step = max(1, len(todo_rows) // (8 * workers))
+                chunks = [(extend_rows, todo_rows[lo:lo + step]) for lo in range(0, len(todo_rows), step)]
+                for t, out in enumerate(pool.imap(_extend_chunk, chunks)):
+                    for r_, n_ in out:
+                        parts_r.append(r_)
+                        parts_n.append(n_)
+                    if log and time.perf_counter() - tick > 30:
+                        tick = time.perf_counter()
+                        log(f"level {depth}: extended {min(len(bad), (t + 1) * step)} of {len(bad)} configurations")
+            elif not last:
+                for t, i in enumerate(bad):
+                    r_, n_ = extend_rows(rows[i, :ns[i]])
+                    parts_r.append(r_)
+                    parts_n.append(n_)
+                    if log and time.perf_counter() - tick > 30:
+                        tick = time.perf_counter()
+                        log(f"level {depth}: extended {t + 1} of {len(bad)} configurations")
+            t4 = time.perf_counter()
+            lv = Level(depth, len(fresh), len(todo), found, len(bad), max(norms[k] for k in set(ckeys)),
+                       max((norms[ckeys[i]] for i in bad), default=-1),
+                       {"dedup": t1 - t0, "canonical": t2 - t1, "search": t3 - t2, "extend": t4 - t3})
+            levels.append(lv)
+            if log:
+                log(f"level {depth}: {lv.configurations} configurations, {lv.new_canonical} new canonical, "
+                    f"{lv.found} found, {lv.extended} without a sorting (3-norm <= {lv.max_bad_three_norm}), "
+                    f"3-norm <= {lv.max_three_norm}, "
+                    + ", ".join(f"{k} {v:.2f}s" for k, v in lv.seconds.items()))
+            if last or not parts_r:
+                break
+            rows, ns = np.concatenate(parts_r), np.concatenate(parts_n)
+    finally:
+        if pool is not None:
+            pool.close()
+            pool.join()
     return Exploration(levels, {tuple(b for b in k if b != PAD): v for k, v in verdicts.items()}, sortings,
                        truncated)
 
@@ -685,6 +715,8 @@ def main(argv=None):
     ap.add_argument("--max-configurations", type=int, default=1 << 21)
     ap.add_argument("--queue-mb", type=int, default=16384)
     ap.add_argument("--dedup-log2", type=int, default=26)
+    ap.add_argument("--workers", type=int, default=min(16, os.cpu_count() or 1),
+                    help="host processes that extend a level's configurations")
     args = ap.parse_args(argv)
     roots = DFS_ROOTS if args.tree == "dfs" else [c for c, _ in BAD_SMALL_COMPONENTS]
     ext = dfs_extension_rows if args.tree == "dfs" else combination_extension_rows
@@ -693,7 +725,7 @@ def main(argv=None):
         t0 = time.perf_counter()
         res = explore(roots, ext, args.levels, args.max_ratio, args.max_depth,
                       max_configurations=args.max_configurations,
-                      log=lambda m: print(m, file=sys.stderr, flush=True))
+                      log=lambda m: print(m, file=sys.stderr, flush=True), workers=args.workers)
         wall = time.perf_counter() - t0
     finally:
         G.destroy_slots()

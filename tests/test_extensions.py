@@ -297,6 +297,17 @@ def test_dfs_tree_three_levels_on_the_gpu(slots):
 
 
 @pytest.mark.gpu
+def test_dfs_tree_extended_on_worker_processes_equals_one_process(slots):
+    # levels extended on four host processes: the same levels, canonical
+    # forms and verdicts as one process (chunks joined in order)
+    one = E.explore(E.DFS_ROOTS, E.dfs_extension_rows, 4, 1.375, 11, slot=slots[0])
+    four = E.explore(E.DFS_ROOTS, E.dfs_extension_rows, 4, 1.375, 11, slot=slots[0], workers=4)
+    assert [vars(l)["configurations"] for l in one.levels] == [vars(l)["configurations"] for l in four.levels]
+    assert [l.new_canonical for l in one.levels] == [l.new_canonical for l in four.levels]
+    assert one.verdicts == four.verdicts
+
+
+@pytest.mark.gpu
 def test_combination_tree_runs_to_completion_on_the_gpu(slots):
     """index.html:63: no combination of bad small components with 3-norm
     greater than 7 was extended, and the search over combinations ends."""
