@@ -125,8 +125,9 @@ def last_launch_stats(slot: int) -> dict:
 class BatchResult:
     found: np.ndarray          # (count,) bool
     depth: np.ndarray          # (count,) int
-    moves: list                # per case: list of (i, j, k) positions
+    moves: list                # per case: list of (i, j, k) positions (None: see moves_array)
     counters: dict             # name -> (count,) uint64 array
+    moves_array: Optional[np.ndarray] = None   # (count, MAX_DEPTH, 3) positions, rows beyond depth unset
 
 
 def search_batch(slot: int, pis: Sequence[Sequence[int]], spis: Sequence[Sequence[int]],
@@ -149,11 +150,12 @@ _CASE_DTYPE = np.dtype(_lib.CaseResult)
 
 
 def search_batch_rows(slot: int, P: np.ndarray, S: np.ndarray, ns, initial_even_cycles, min_rates, max_moves,
-                      full_sorting=None, flags: int = 0) -> BatchResult:
+                      full_sorting=None, flags: int = 0, move_lists: bool = True) -> BatchResult:
     """search_batch on pre-packed rows: P and S (count, 64) uint8 (pi and spi
     of configuration c in their first ns[c] entries); moves are position
     triples, read with the found flags, depths and counters through a numpy
-    view of the result array."""
+    view of the result array.  move_lists=False leaves moves None and gives
+    the positions only as moves_array (no per-configuration Python lists)."""
     count = len(ns)
     P = np.ascontiguousarray(P, np.uint8)
     S = np.ascontiguousarray(S, np.uint8)
@@ -175,10 +177,10 @@ def search_batch_rows(slot: int, P: np.ndarray, S: np.ndarray, ns, initial_even_
     arr = np.frombuffer(out, dtype=_CASE_DTYPE, count=count)
     found = arr["found"] != 0
     depth = arr["depth"].astype(np.int32)
-    mv = arr["moves"].reshape(count, _lib.MAX_DEPTH, 3)
-    moves = [[tuple(m) for m in mv[c, :max(int(depth[c]), 0)].tolist()] for c in range(count)]
+    mv = np.array(arr["moves"]).reshape(count, _lib.MAX_DEPTH, 3)
+    moves = [[tuple(m) for m in mv[c, :max(int(depth[c]), 0)].tolist()] for c in range(count)] if move_lists else None
     counters = {name: arr["counters"][name].astype(np.uint64) for name, _ in _lib.Counters._fields_}
-    return BatchResult(found, depth, moves, counters)
+    return BatchResult(found, depth, moves, counters, mv)
 
 
 class GPUSortingSearch:

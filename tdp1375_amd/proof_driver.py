@@ -289,7 +289,7 @@ def sort_rows(slot: int, rows: np.ndarray, ns, max_ratio: float, max_depth: int,
         if todo.size == 0:
             break
         res = search_batch_rows(slot, ident[todo], rows[todo], ns[todo], e0[todo], min_rate, rungs[todo, rung], 0,
-                                flags=flags)
+                                flags=flags, move_lists=False)
         launches += 1
         tried[todo] = rungs[todo, rung]
         hit = np.flatnonzero(res.found)
@@ -299,7 +299,7 @@ def sort_rows(slot: int, rows: np.ndarray, ns, max_ratio: float, max_depth: int,
         depth = res.depth[hit]
         for d in np.unique(depth):
             grp = hit[depth == d]
-            pos = np.asarray([res.moves[q] for q in grp], np.int64).reshape(len(grp), int(d), 3)
+            pos = res.moves_array[grp, :int(d)].astype(np.int64)
             cur = ident[todo[grp]].astype(np.int64)
             sym = np.empty_like(pos)
             for t in range(int(d)):

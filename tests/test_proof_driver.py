@@ -133,12 +133,16 @@ def test_batched_ladder_on_the_oracle_equals_the_sequential_ladder(monkeypatch):
     from tdp1375_amd.gpu_sorting_search import BatchResult
     calls = []
 
-    def fake_rows(slot, P, S, ns, e0, min_rate, depths, full, flags=0):
+    def fake_rows(slot, P, S, ns, e0, min_rate, depths, full, flags=0, move_lists=True):
         calls.append(len(ns))
         res = [oracle.search(list(P[c, :ns[c]]), list(S[c, :ns[c]]), int(e0[c]), min_rate, int(depths[c]))
                for c in range(len(ns))]
+        arr = np.zeros((len(ns), 15, 3), np.int64)
+        for c, r in enumerate(res):
+            if r.moves:
+                arr[c, :len(r.moves)] = np.asarray(r.moves)
         return BatchResult(np.array([r.found for r in res]), np.array([r.depth for r in res], np.int32),
-                           [r.moves for r in res], {})
+                           [r.moves for r in res] if move_lists else None, {}, arr)
 
     monkeypatch.setattr(PD, "search_batch_rows", fake_rows)
     data = load_cases()
