@@ -623,15 +623,15 @@ def explore(roots: Sequence[Cycles], extend_rows: Callable[[Sequence[int]], Tupl
         for depth in range(max_levels):
             t0 = time.perf_counter()
             view = np.ascontiguousarray(rows).view(np.dtype((np.void, ROW_WIDTH))).ravel()
-            keys = view.tolist()
             first = np.sort(np.unique(view, return_index=True)[1])
-            fresh = [int(i) for i in first if keys[i] not in seen]
+            fresh_keys = [(int(i), k) for i, k in zip(first.tolist(), view[first].tolist()) if k not in seen]
+            fresh = [i for i, _ in fresh_keys]
             if not fresh:
                 break
             if len(fresh) > max_configurations:
                 truncated = True
                 break
-            seen.update(keys[i] for i in fresh)
+            seen.update(k for _, k in fresh_keys)
             rows, ns = rows[fresh], ns[fresh]
             t1 = time.perf_counter()
             _, _, _, canon = K.canonical_rows(rows, ns)
@@ -651,7 +651,7 @@ def explore(roots: Sequence[Cycles], extend_rows: Callable[[Sequence[int]], Tupl
                     verdicts[k] = bool(res.found[q])
                     norms[k] = int(three[q])
                     if res.found[q]:
-                        sortings[tuple(R[q, :N[q]].tolist())] = res.moves[q]
+                        sortings[tuple(k[:N[q]])] = res.moves[q]
                         found += 1
             t3 = time.perf_counter()
             bad = [i for i, k in enumerate(ckeys) if not verdicts[k]]
@@ -696,8 +696,13 @@ def explore(roots: Sequence[Cycles], extend_rows: Callable[[Sequence[int]], Tupl
         if pool is not None:
             pool.close()
             pool.join()
-    return Exploration(levels, {tuple(b for b in k if b != PAD): v for k, v in verdicts.items()}, sortings,
-                       truncated)
+    pad = bytes([PAD])
+
+    def symbols(k: bytes) -> Tuple[int, ...]:
+        n = k.find(pad)
+        return tuple(k if n < 0 else k[:n])
+
+    return Exploration(levels, {symbols(k): v for k, v in verdicts.items()}, sortings, truncated)
 
 
 def main(argv=None):

@@ -300,13 +300,15 @@ def sort_rows(slot: int, rows: np.ndarray, ns, max_ratio: float, max_depth: int,
         for d in np.unique(depth):
             grp = hit[depth == d]
             pos = res.moves_array[grp, :int(d)].astype(np.int64)
-            cur = ident[todo[grp]].astype(np.int64)
-            sym = np.empty_like(pos)
+            # only the first max n positions can move
+            width = int(ns[todo[grp]].max())
+            cur = ident[todo[grp], :width]
+            sym = np.empty(pos.shape, np.uint8)
             for t in range(int(d)):
                 sym[:, t] = np.take_along_axis(cur, pos[:, t], 1)
                 cur = _block_swap_rows(cur, pos[:, t, 0], pos[:, t, 1], pos[:, t, 2])
-            for q, c in enumerate(todo[grp]):
-                moves[c] = [tuple(m) for m in sym[q].tolist()]
+            for c, seq in zip(todo[grp].tolist(), sym.tolist()):
+                moves[c] = list(map(tuple, seq))
     return LadderResult(found, tried, moves, launches)
 
 
