@@ -12,7 +12,7 @@ labelled by the move the sequence took (learned.build_dataset, built on
 
 Networks trained on the same data differ (initialisation and batch order,
 --seed, --seed + 1, ...), and so do the learned search's times with them by
-10-20 %.  --candidates networks are trained and each is run as the learned
+10-40 %.  --candidates networks are trained and each is run as the learned
 ordering over a validation batch of the depth-11 generator (seed 19 by
 default, in neither the training nor the benchmark batches, with the cached
 learned-ordering tuning); the one with the least median search time is kept.
@@ -86,11 +86,11 @@ def main(argv=None):
     ap.add_argument("--epochs", type=int, default=16)
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--batch-groups", type=int, default=64)
-    ap.add_argument("--seed", type=int, default=5)
+    ap.add_argument("--seed", type=int, default=8)
     ap.add_argument("--seeds", default="11,12,13,14,15,16,17,18",
                     help="generator seeds of the depth-11 batches (the benchmark uses 3)")
     ap.add_argument("--workers", type=int, default=min(16, os.cpu_count() or 1))
-    ap.add_argument("--candidates", type=int, default=3, help="networks trained (training seeds --seed, --seed + 1, ...)")
+    ap.add_argument("--candidates", type=int, default=5, help="networks trained (training seeds --seed, --seed + 1, ...)")
     ap.add_argument("--validation-seed", type=int, default=19,
                     help="generator seed of the depth-11 batch that picks among the candidates")
     ap.add_argument("--validation-steps", type=int, default=5)
