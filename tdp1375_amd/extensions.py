@@ -30,6 +30,7 @@ import math
 import os
 import time
 from dataclasses import dataclass, field
+from itertools import compress
 from multiprocessing import Pool
 from typing import Callable, Dict, List, Optional, Sequence, Set, Tuple
 
@@ -624,21 +625,23 @@ def explore(roots: Sequence[Cycles], extend_rows: Callable[[Sequence[int]], Tupl
             t0 = time.perf_counter()
             view = np.ascontiguousarray(rows).view(np.dtype((np.void, ROW_WIDTH))).ravel()
             first = np.sort(np.unique(view, return_index=True)[1])
-            fresh_keys = [(int(i), k) for i, k in zip(first.tolist(), view[first].tolist()) if k not in seen]
-            fresh = [i for i, _ in fresh_keys]
-            if not fresh:
+            keys = view[first].tolist()
+            unseen = ~np.fromiter(map(seen.__contains__, keys), bool, len(keys))
+            fresh = first[unseen]
+            if not len(fresh):
                 break
             if len(fresh) > max_configurations:
                 truncated = True
                 break
-            seen.update(k for _, k in fresh_keys)
+            seen.update(compress(keys, unseen.tolist()))
             rows, ns = rows[fresh], ns[fresh]
             t1 = time.perf_counter()
             _, _, _, canon = K.canonical_rows(rows, ns)
             canon[np.arange(ROW_WIDTH)[None, :] >= ns[:, None]] = PAD
             ckeys = _keys(canon)
             t2 = time.perf_counter()
-            todo = sorted({k for k in ckeys if k not in verdicts})
+            cset = set(ckeys)
+            todo = sorted(cset.difference(verdicts))
             found = 0
             for lo in range(0, len(todo), SEARCH_BATCH_MAX):
                 part = todo[lo:lo + SEARCH_BATCH_MAX]
@@ -647,14 +650,14 @@ def explore(roots: Sequence[Cycles], extend_rows: Callable[[Sequence[int]], Tupl
                 e0 = D.odd_cycle_counts(R, N)
                 three = (N - e0) // 2
                 res = D.sort_rows(slot, R, N, max_ratio, max_depth, e0=e0, norms=three)
-                for q, k in enumerate(part):
-                    verdicts[k] = bool(res.found[q])
-                    norms[k] = int(three[q])
-                    if res.found[q]:
-                        sortings[tuple(k[:N[q]])] = res.moves[q]
-                        found += 1
+                verdicts.update(zip(part, res.found.tolist()))
+                norms.update(zip(part, three.tolist()))
+                hit = np.flatnonzero(res.found).tolist()
+                width = N.tolist()
+                sortings.update((tuple(part[q][:width[q]]), res.moves[q]) for q in hit)
+                found += len(hit)
             t3 = time.perf_counter()
-            bad = [i for i, k in enumerate(ckeys) if not verdicts[k]]
+            bad = np.flatnonzero(~np.fromiter(map(verdicts.__getitem__, ckeys), bool, len(ckeys))).tolist()
             last = depth + 1 == max_levels
             parts_r, parts_n = [], []
             tick = time.perf_counter()
@@ -680,7 +683,7 @@ def explore(roots: Sequence[Cycles], extend_rows: Callable[[Sequence[int]], Tupl
                         tick = time.perf_counter()
                         log(f"level {depth}: extended {t + 1} of {len(bad)} configurations")
             t4 = time.perf_counter()
-            lv = Level(depth, len(fresh), len(todo), found, len(bad), max(norms[k] for k in set(ckeys)),
+            lv = Level(depth, len(fresh), len(todo), found, len(bad), max(map(norms.__getitem__, cset)),
                        max((norms[ckeys[i]] for i in bad), default=-1),
                        {"dedup": t1 - t0, "canonical": t2 - t1, "search": t3 - t2, "extend": t4 - t3})
             levels.append(lv)

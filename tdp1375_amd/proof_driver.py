@@ -244,12 +244,17 @@ def odd_cycle_counts(rows: np.ndarray, ns) -> np.ndarray:
     pointer doubling, cycle lengths by counting symbols per minimum."""
     rows = np.asarray(rows)
     ns = np.asarray(ns)
-    K, W = rows.shape
+    K = rows.shape[0]
+    if K == 0:
+        return np.zeros(0, np.int64)
+    # columns past the longest row are fixed points of every row: drop them
+    W = max(1, int(ns.max()))
+    rows = rows[:, :W]
     x = np.arange(W)
     inside = x[None, :] < ns[:, None]
-    q = np.where(inside, rows.astype(np.int64), x[None, :])
+    q = np.where(inside, rows.astype(np.intp), x[None, :])
     m = np.broadcast_to(x, (K, W)).copy()
-    for _ in range(6):
+    for _ in range(max(1, (W - 1).bit_length())):
         m = np.minimum(m, np.take_along_axis(m, q, 1))
         q = np.take_along_axis(q, q, 1)
     length = np.bincount((np.arange(K)[:, None] * W + m).ravel(), minlength=K * W).reshape(K, W)
@@ -307,8 +312,13 @@ def sort_rows(slot: int, rows: np.ndarray, ns, max_ratio: float, max_depth: int,
             for t in range(int(d)):
                 sym[:, t] = np.take_along_axis(cur, pos[:, t], 1)
                 cur = _block_swap_rows(cur, pos[:, t, 0], pos[:, t, 1], pos[:, t, 2])
-            for c, seq in zip(todo[grp].tolist(), sym.tolist()):
-                moves[c] = list(map(tuple, seq))
+            # one (a, b, c) tuple per move straight from the three symbol
+            # columns, then each configuration's d consecutive moves
+            flat = sym.reshape(-1, 3)
+            tuples = list(zip(flat[:, 0].tolist(), flat[:, 1].tolist(), flat[:, 2].tolist()))
+            step = int(d)
+            for c, lo in zip(todo[grp].tolist(), range(0, len(tuples), step)):
+                moves[c] = tuples[lo:lo + step]
     return LadderResult(found, tried, moves, launches)
 
 

@@ -125,6 +125,31 @@ def test_search_for_sorting_walks_the_ladder():
     assert gpu.calls[0] == 4 and gpu.calls[-1] == 8
 
 
+def test_odd_cycle_counts_by_walking_every_cycle():
+    """odd_cycle_counts (pointer doubling over the widest row's columns) on
+    random one-line rows of n = 1..64, mixed and equal widths, against
+    walking each cycle; fixed points are odd cycles."""
+    rng = np.random.default_rng(7)
+    for trial in range(60):
+        K = int(rng.integers(1, 40))
+        ns = rng.integers(1, 65, K) if trial % 2 else np.full(K, int(rng.integers(1, 12)))
+        rows = np.tile(np.arange(64, dtype=np.uint8), (K, 1))
+        want = []
+        for c in range(K):
+            rows[c, :ns[c]] = rng.permutation(int(ns[c]))
+            seen, odd = set(), 0
+            for s0 in range(int(ns[c])):
+                length, x = 0, s0
+                while x not in seen:
+                    seen.add(x)
+                    x = int(rows[c, x])
+                    length += 1
+                odd += length % 2
+            want.append(odd)
+        assert PD.odd_cycle_counts(rows, ns).tolist() == want, trial
+    assert PD.odd_cycle_counts(np.zeros((0, 64), np.uint8), np.zeros(0, np.int32)).size == 0
+
+
 def test_batched_ladder_on_the_oracle_equals_the_sequential_ladder(monkeypatch):
     """sort_batch (rows, one launch per rung, vectorised odd-cycle counts,
     bounds and positions -> symbols) with the oracle standing in for the
