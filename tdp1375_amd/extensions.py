@@ -722,7 +722,9 @@ def main(argv=None):
     ap.add_argument("--max-depth", type=int, default=11)
     ap.add_argument("--max-configurations", type=int, default=1 << 21)
     ap.add_argument("--queue-mb", type=int, default=16384)
-    ap.add_argument("--dedup-log2", type=int, default=26)
+    # 2^28 entries (19.3 GB of tags and keys): the exhaustive depth-11 rungs of
+    # the unsatisfiable configurations re-expand fewer states than with 2^26
+    ap.add_argument("--dedup-log2", type=int, default=28)
     ap.add_argument("--workers", type=int, default=min(16, os.cpu_count() or 1),
                     help="host processes that extend a level's configurations")
     args = ap.parse_args(argv)
