@@ -30,7 +30,7 @@ from __future__ import annotations
 
 import os
 import threading
-from concurrent.futures import Future
+from concurrent.futures import Future, ThreadPoolExecutor
 from dataclasses import dataclass
 from typing import Callable, Dict, Iterable, List, Optional, Sequence, Tuple
 
@@ -289,36 +289,46 @@ def sort_rows(slot: int, rows: np.ndarray, ns, max_ratio: float, max_depth: int,
     tried = np.zeros(count, np.int32)
     moves: List[Optional[List[Move]]] = [None] * count
     launches = 0
-    for rung in range(3):
-        todo = np.flatnonzero(~found & (rung < nrungs))
-        if todo.size == 0:
-            break
-        res = search_batch_rows(slot, ident[todo], rows[todo], ns[todo], e0[todo], min_rate, rungs[todo, rung], 0,
-                                flags=flags, move_lists=False)
-        launches += 1
-        tried[todo] = rungs[todo, rung]
-        hit = np.flatnonzero(res.found)
-        found[todo[hit]] = True
-        # positions -> symbols, replayed on pi for all found configurations of
-        # each length at once (GPUSortingSearch.java:65-87)
-        depth = res.depth[hit]
-        for d in np.unique(depth):
-            grp = hit[depth == d]
-            pos = res.moves_array[grp, :int(d)].astype(np.int64)
+
+    def convert(idx: np.ndarray, depth: np.ndarray, pos_all: np.ndarray, width_all: np.ndarray) -> None:
+        """Positions -> symbols, replayed on pi (the identity) for all found
+        configurations of each length at once (GPUSortingSearch.java:65-87)."""
+        for d in np.unique(depth).tolist():
+            sel = depth == d
+            pos = pos_all[sel, :d].astype(np.int64)
             # only the first max n positions can move
-            width = int(ns[todo[grp]].max())
-            cur = ident[todo[grp], :width]
+            width = int(width_all[sel].max())
+            cur = np.ascontiguousarray(np.broadcast_to(np.arange(width, dtype=np.uint8), (len(pos), width)))
             sym = np.empty(pos.shape, np.uint8)
-            for t in range(int(d)):
+            for t in range(d):
                 sym[:, t] = np.take_along_axis(cur, pos[:, t], 1)
                 cur = _block_swap_rows(cur, pos[:, t, 0], pos[:, t, 1], pos[:, t, 2])
             # one (a, b, c) tuple per move straight from the three symbol
             # columns, then each configuration's d consecutive moves
             flat = sym.reshape(-1, 3)
             tuples = list(zip(flat[:, 0].tolist(), flat[:, 1].tolist(), flat[:, 2].tolist()))
-            step = int(d)
-            for c, lo in zip(todo[grp].tolist(), range(0, len(tuples), step)):
-                moves[c] = tuples[lo:lo + step]
+            for c, lo in zip(idx[sel].tolist(), range(0, len(tuples), d)):
+                moves[c] = tuples[lo:lo + d]
+
+    # a rung's sequences are converted on a host thread while the GPU runs the
+    # next rung (the library call releases the interpreter lock); the thread
+    # gets copies, since the next call reuses the result buffer
+    with ThreadPoolExecutor(1) as converter:
+        pending = []
+        for rung in range(3):
+            todo = np.flatnonzero(~found & (rung < nrungs))
+            if todo.size == 0:
+                break
+            res = search_batch_rows(slot, ident[todo], rows[todo], ns[todo], e0[todo], min_rate, rungs[todo, rung],
+                                    0, flags=flags, move_lists=False)
+            launches += 1
+            tried[todo] = rungs[todo, rung]
+            hit = np.flatnonzero(res.found)
+            found[todo[hit]] = True
+            pending.append(converter.submit(convert, todo[hit], np.asarray(res.depth)[hit],
+                                            np.asarray(res.moves_array)[hit], ns[todo[hit]]))
+        for job in pending:
+            job.result()
     return LadderResult(found, tried, moves, launches)
 
 
